@@ -1,0 +1,165 @@
+/*
+ * mhd_hip.h - C ABI of the MI355X (gfx950) "hip" backend of the 2-D GLM-MHD solver.
+ *
+ * Drop-in boundary: every entry point below replaces one member of the reference's device
+ * backend interface (include/mhd/gpu/device_solver.hpp:57-75) or its factory/discovery helpers
+ * (device_solver.hpp:26-111), with plain pointers, sizes and integer status codes so that a C++
+ * DeviceSolver adapter, a pybind11 module or ctypes can bind it. Paths cited are relative to the
+ * reference repository root.
+ *
+ * Status codes: MHD_HIP_OK on success; MHD_HIP_EINVAL where the reference throws
+ * std::invalid_argument; MHD_HIP_ERUNTIME where it throws std::runtime_error (device failure,
+ * max_steps reached before t_end); MHD_HIP_ENODEV when no usable GPU is present.
+ * mhd_hip_last_error() returns the calling thread's message for the last failure.
+ *
+ * State layout on the host side of upload/download: nine arrays (rho, mx, my, mz, energy, bx,
+ * by, bz, psi), each (ny + 2 ng) * (nx + 2 ng) doubles, row-major, ghosts included
+ * (include/state/state_field.hpp:9-45).
+ */
+#ifndef MHD_HIP_H
+#define MHD_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MHD_HIP_ABI_VERSION 1
+
+enum mhd_hip_status {
+    MHD_HIP_OK = 0,
+    MHD_HIP_EINVAL = 1,
+    MHD_HIP_ERUNTIME = 2,
+    MHD_HIP_ENODEV = 3
+};
+
+/* mhd::Precision (device_solver.hpp:28) */
+enum mhd_hip_precision { MHD_HIP_FP64 = 0, MHD_HIP_FP32 = 1 };
+
+/* BoundaryConditionType (include/boundary_conditions/boundary_conditions.hpp:8) */
+enum mhd_hip_bc { MHD_HIP_PERIODIC = 0, MHD_HIP_REFLECTIVE = 1, MHD_HIP_OUTFLOW = 2 };
+
+/* SlopeLimiter (include/mhd/mhd_types.hpp:44) */
+enum mhd_hip_limiter { MHD_HIP_MINMOD = 0, MHD_HIP_MC = 1 };
+
+/* Grid2D (include/grid/grid.hpp:5-30): interior nx x ny, ng ghosts per side. */
+typedef struct mhd_hip_grid {
+    int nx, ny, ng;
+    double x_min, x_max, y_min, y_max;
+} mhd_hip_grid;
+
+/* SolveParams (include/mhd/solve_types.hpp:8-16) with NonIdealConfig flattened
+ * (include/physics_config/mhd_config.hpp:13-24). */
+typedef struct mhd_hip_params {
+    double t_end;
+    double cfl;
+    double gamma;
+    double glm_alpha;
+    int limiter;
+    int ohmic, hall, ambipolar;
+    double eta_ohm, eta_hall, eta_ambipolar;
+    int max_steps;
+} mhd_hip_params;
+
+/* SolveResult (include/mhd/solve_types.hpp:18-22) */
+typedef struct mhd_hip_result {
+    double t;
+    int steps;
+    double c_h;
+} mhd_hip_result;
+
+typedef struct mhd_hip_solver mhd_hip_solver;
+
+/* Replaces cuda_available()/metal_available() (device_solver.hpp:77-78): 1 if a gfx950 device
+ * is present and the kernels load, else 0. Never fails. */
+int mhd_hip_available(void);
+
+/* Thread-local message of the last failing call on this thread ("" if none). */
+const char* mhd_hip_last_error(void);
+
+/* Replaces make_cuda_solver(grid, precision) (device_solver.hpp:81-82, cuda_solver.cu:786-798):
+ * allocates the device state for `grid` on the current HIP device. EINVAL for ng < 2 or
+ * nx, ny < 1; ENODEV without a device. */
+int mhd_hip_create(const mhd_hip_grid* grid, int precision, mhd_hip_solver** out);
+
+/* Replaces ~DeviceSolver. Null is accepted. */
+void mhd_hip_destroy(mhd_hip_solver* s);
+
+/* Replaces DeviceSolver::device_name() (device_solver.hpp:62). Writes a NUL-terminated name. */
+int mhd_hip_device_name(const mhd_hip_solver* s, char* buf, size_t len);
+
+/* Replaces DeviceSolver::precision() (device_solver.hpp:63). */
+int mhd_hip_precision(const mhd_hip_solver* s);
+
+/* Replaces DeviceSolver::configure(bc, mask, reflective_walls) (device_solver.hpp:66-67).
+ * mask: ny * nx bytes, row-major over the interior, 1 = fluid (include/mhd/mask.hpp:17-20);
+ * NULL (or all fluid) means no solids. EINVAL for Reflective boundaries. */
+int mhd_hip_configure(mhd_hip_solver* s, int bc_x, int bc_y, const uint8_t* mask, size_t mask_len,
+                      int reflective_walls);
+
+/* Replaces DeviceSolver::upload(const StateField&) (device_solver.hpp:70). comps[c] points at
+ * plane c of the full state; n must equal (nx + 2 ng) * (ny + 2 ng). */
+int mhd_hip_upload(mhd_hip_solver* s, const double* const comps[9], size_t n);
+
+/* Replaces DeviceSolver::download(StateField&) (device_solver.hpp:71). */
+int mhd_hip_download(mhd_hip_solver* s, double* const comps[9], size_t n);
+
+/* Replaces DeviceSolver::solve(const SolveParams&) (device_solver.hpp:74): advances the device
+ * state by the relative time params->t_end with the contract of ::solve()
+ * (include/mhd/time_integrator.hpp:62-106): EINVAL for t_end < 0 or cfl <= 0, ERUNTIME when
+ * max_steps is reached before t_end; ghosts and solid cells are filled at the end. */
+int mhd_hip_solve(mhd_hip_solver* s, const mhd_hip_params* params, mhd_hip_result* result);
+
+/* Fixed step count (benchmarks, dt-sequence checks): exactly `steps` steps of the scheme, as
+ * ::solve() with t_end = +inf and max_steps = steps, without the max_steps error. The
+ * reference's solve loop (time_integrator.hpp:84-95) with the loop bound as the only change. */
+int mhd_hip_advance_steps(mhd_hip_solver* s, const mhd_hip_params* params, int steps,
+                          mhd_hip_result* result);
+
+/* Records the dt of the first `capacity` steps of each later solve/advance call
+ * (capacity 0 disables). mhd_hip_dt_log copies min(count, capacity) values and returns the
+ * number copied through *count. */
+int mhd_hip_set_dt_log(mhd_hip_solver* s, int capacity);
+int mhd_hip_dt_log(mhd_hip_solver* s, double* out, int capacity, int* count);
+
+/* Ghost fill and/or solid-wall fill of the device state, as MHDSolver::initialize and set_mask
+ * do on the host state (include/mhd/solver_api.hpp:148-154, :286-289). `what` is a bit set:
+ * MHD_HIP_FILL_GHOSTS, MHD_HIP_FILL_WALLS (walls only with an active mask and reflective walls);
+ * ghosts are filled before walls. */
+#define MHD_HIP_FILL_GHOSTS 1
+#define MHD_HIP_FILL_WALLS 2
+int mhd_hip_fill_boundaries(mhd_hip_solver* s, double gamma, int what);
+
+/* Seconds of device time spent in the last solve/advance call (HIP events on the solver's
+ * stream around the step loop, excluding upload/download). */
+int mhd_hip_last_solve_seconds(const mhd_hip_solver* s, double* seconds);
+
+/* Host-side initial condition (include/initial_condition/inititial_condition.hpp:28-270):
+ * writes the conserved interior of the nine planes for a reference IC name (uniform, sine,
+ * blast, orszag_tang, sod, brio_wu, rotor, kelvin_helmholtz / kh, alfven / alfven_wave, with
+ * the reference aliases) or one of this backend's parametrised workloads:
+ *   "orszag_tang_b0"   Orszag-Tang with B scaled by `b0` (b0 = 1/sqrt(4 pi) for the standard
+ *                      normalisation)
+ *   "rotor_b0"         rotor with Bx = b0 * 5/sqrt(4 pi)
+ *   "kh_tanh"          tanh shear layer, vx = +/-0.5, width 0.05, Bx = 0.1, vy seeded noise
+ *                      of amplitude 1e-2 from `seed`
+ *   "alfven_oblique"   circularly polarised Alfven wave, amplitude 0.1, along angle atan(2)
+ * `b0` and `seed` are ignored by the reference names. Ghost cells are not written.
+ * EINVAL for an unknown name. Runs on the CPU; needs no device. */
+int mhd_hip_init_ic(const char* name, const mhd_hip_grid* grid, double gamma, double b0,
+                    uint64_t seed, double* const comps[9]);
+
+/* Host-side solid-wall plan (include/mhd/mask.hpp:85-149): for each solid interior cell in
+ * row-major order, (destination cell, first fluid interior neighbour in the order +x, -x, +y,
+ * -y or -1, axis 0/1). plan must hold 3 * (number of solid cells) entries; returns the count
+ * through *count. Runs on the CPU; needs no device. */
+int mhd_hip_wall_plan(const mhd_hip_grid* grid, const uint8_t* mask, size_t mask_len,
+                      int64_t* plan, size_t plan_capacity, size_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MHD_HIP_H */

@@ -1,0 +1,531 @@
+// HIP kernels of one SSP-RK2 step of the GLM-MHD solver (gfx950, wave64).
+//
+// Kernels
+//   k_ctrl_init   zero the device control block (no memset nodes, graph-capture safe)
+//   k_cfl         max signal speed over fluid interior cells -> ctrl.chmax[0]
+//   k_ghost_fill  all ghost cells from their interior sources (x-then-y fill, corners exact)
+//   k_wall_fill   reflective slip fill of solid cells from a precomputed neighbour plan
+//   k_stage<S>    fused RK stage S in {1,2}: MUSCL slopes, HLL x/y faces, non-ideal E,
+//                 flux difference, GLM psi damping, solid zero, RK combine; stage 2 also
+//                 reduces the next step's CFL signal speed (wave shuffle + one atomic/wave).
+//
+// Step bookkeeping stays on the device: ctrl holds (t, steps, c_h, chmax) double-buffered by
+// step parity; every stage kernel recomputes dt from chmax[p] with the reference formula
+// (include/mhd/cfl.hpp:77-89, include/mhd/time_integrator.hpp:89) and returns at once when the
+// step is inactive (t >= t_end or steps >= max_steps), so surplus steps in a launched batch
+// are no-ops and a batch can run ahead of the host.
+#pragma once
+
+#include "mhd_physics.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace mhdhip {
+
+constexpr int kWave = 64;
+constexpr int kStripCells = kWave - 4;  // cells per wave strip: 2 halo lanes left, 2 right
+constexpr int kWavesPerBlock = 4;
+constexpr int kBlock = kWave * kWavesPerBlock;
+
+struct Ctrl {
+    double t[2];
+    double ch_last[2];
+    unsigned long long chmax[2];
+    int steps[2];
+    int bad_dt;  // set when an active step got dt <= 0 (the reference throws there)
+    int pad;
+};
+
+struct Geom {
+    int nx, ny, ng, nxt, nyt;
+    long long plane;  // elements per component plane, nxt * nyt
+    int bc_x, bc_y;   // 0 periodic, 2 outflow
+};
+
+struct Physics {
+    double gamma, gm1, inv_gm1;
+    double cfl, inv_dx, inv_dy, inv_2dx, inv_2dy;
+    double glm_alpha, h;
+    double dt_diff;  // cfl h^2 / (4 eta_max); only used when has_diff
+    int has_diff;
+    double t_end;
+    int max_steps;
+    NonIdeal ni;
+};
+
+struct Tiling {
+    int strips;    // 60-column strips across nx
+    int seg_rows;  // rows marched by one wave
+    int segs;      // row segments across ny
+};
+
+struct StepState {
+    double t, dt, c_h;
+    int steps;
+    bool active;
+};
+
+// dt for the step with parity p: include/mhd/cfl.hpp:77-89, time_integrator.hpp:85-90
+__device__ __forceinline__ StepState step_state(const Ctrl* ctrl, const Physics& ph, int p)
+{
+    StepState s;
+    s.t = ctrl->t[p];
+    s.steps = ctrl->steps[p];
+    s.c_h = __longlong_as_double(static_cast<long long>(ctrl->chmax[p]));
+    double dt = (s.c_h <= 0.0) ? __builtin_huge_val() : ph.cfl / (s.c_h * ph.inv_dx + s.c_h * ph.inv_dy);
+    if (ph.has_diff) {
+        dt = vmin(dt, ph.dt_diff);
+    }
+    s.dt = vmin(dt, ph.t_end - s.t);
+    // include/mhd/time_integrator.hpp:42-44: a step with dt <= 0 is an error, never executed
+    s.active = (s.t < ph.t_end) && (s.steps < ph.max_steps) && (s.dt > 0.0);
+    return s;
+}
+
+// step not yet finished by time or count (an inactive step with this true had dt <= 0)
+__device__ __forceinline__ bool ctrl_pending(const Ctrl* ctrl, const Physics& ph, int p)
+{
+    return ctrl->t[p] < ph.t_end && ctrl->steps[p] < ph.max_steps;
+}
+
+template <typename R>
+__device__ __forceinline__ R wave_max(R v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const R o = __shfl_xor(v, off);
+        v = vmax(v, o);
+    }
+    return v;
+}
+
+__device__ __forceinline__ void atomic_max_speed(unsigned long long* slot, double v)
+{
+    // non-negative doubles order like their bit patterns
+    atomicMax(slot, static_cast<unsigned long long>(__double_as_longlong(v)));
+}
+
+template <typename R>
+__global__ void k_ctrl_init(Ctrl* ctrl)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ctrl->t[0] = 0.0;
+        ctrl->t[1] = 0.0;
+        ctrl->ch_last[0] = 0.0;
+        ctrl->ch_last[1] = 0.0;
+        ctrl->chmax[0] = 0ull;
+        ctrl->chmax[1] = 0ull;
+        ctrl->steps[0] = 0;
+        ctrl->steps[1] = 0;
+        ctrl->bad_dt = 0;
+        ctrl->pad = 0;
+    }
+}
+
+// ---- CFL over the fluid interior: include/mhd/cfl.hpp:46-71 --------------------------------
+template <typename R>
+__global__ __launch_bounds__(256) void k_cfl(const R* __restrict__ U, Geom g, Physics ph,
+                                             const uint8_t* __restrict__ mask, Ctrl* ctrl)
+{
+    const long long ncell = static_cast<long long>(g.nx) * g.ny;
+    R local = R(0);
+    const R gamma = static_cast<R>(ph.gamma);
+    const R gm1 = static_cast<R>(ph.gm1);
+    for (long long k = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; k < ncell;
+         k += static_cast<long long>(gridDim.x) * blockDim.x) {
+        if (mask != nullptr && mask[k] == 0) {
+            continue;
+        }
+        const int jj = static_cast<int>(k / g.nx);
+        const int ii = static_cast<int>(k - static_cast<long long>(jj) * g.nx);
+        const long long cell = static_cast<long long>(jj + g.ng) * g.nxt + (ii + g.ng);
+        R u[kNC];
+#pragma unroll
+        for (int c = 0; c < kNC; ++c) {
+            u[c] = U[c * g.plane + cell];
+        }
+        local = vmax(local, cell_max_speed(u, gamma, gm1));
+    }
+    local = wave_max(local);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        atomic_max_speed(&ctrl->chmax[0], static_cast<double>(local));
+    }
+}
+
+// ---- ghost fill: include/boundary_conditions/boundary_conditions.hpp:33-103 ---------------
+// A ghost's value after the x pass then the y pass is the interior cell at the mapped (i, j).
+// Gate: the ghost and wall fills run inside a step only while that step is active (so surplus
+// no-op steps leave the state untouched); parity < 0 runs unconditionally.
+struct Gate {
+    const Ctrl* ctrl;
+    double t_end;
+    int max_steps;
+    int parity;
+};
+
+__device__ __forceinline__ bool gate_open(const Gate& gt)
+{
+    if (gt.parity < 0) {
+        return true;
+    }
+    return gt.ctrl->t[gt.parity] < gt.t_end && gt.ctrl->steps[gt.parity] < gt.max_steps;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_ghost_fill(R* __restrict__ U, Geom g, Gate gt)
+{
+    if (!gate_open(gt)) {
+        return;
+    }
+    const long long band = static_cast<long long>(g.ng) * g.nxt;
+    const long long total = 2 * band + static_cast<long long>(g.ny) * 2 * g.ng;
+    for (long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; t < total;
+         t += static_cast<long long>(gridDim.x) * blockDim.x) {
+        int i, j;
+        if (t < band) {
+            j = static_cast<int>(t / g.nxt);
+            i = static_cast<int>(t - static_cast<long long>(j) * g.nxt);
+        } else if (t < 2 * band) {
+            const long long t2 = t - band;
+            j = g.ng + g.ny + static_cast<int>(t2 / g.nxt);
+            i = static_cast<int>(t2 % g.nxt);
+        } else {
+            const long long t3 = t - 2 * band;
+            const int w = 2 * g.ng;
+            j = g.ng + static_cast<int>(t3 / w);
+            const int k = static_cast<int>(t3 % w);
+            i = k < g.ng ? k : g.nx + k;
+        }
+        const int si = ghost_source(i, g.nx, g.ng, g.bc_x);
+        const int sj = ghost_source(j, g.ny, g.ng, g.bc_y);
+        const long long dst = static_cast<long long>(j) * g.nxt + i;
+        const long long src = static_cast<long long>(sj) * g.nxt + si;
+#pragma unroll
+        for (int c = 0; c < kNC; ++c) {
+            U[c * g.plane + dst] = U[c * g.plane + src];
+        }
+    }
+}
+
+// ---- solid wall fill: include/mhd/mask.hpp:85-149 ------------------------------------------
+// plan[3k] = destination cell, plan[3k+1] = fluid source cell or -1, plan[3k+2] = axis (0 x, 1 y)
+// Sources are always fluid cells, so every entry is independent of the others.
+template <typename R>
+__global__ __launch_bounds__(256) void k_wall_fill(R* __restrict__ U, Geom g,
+                                                   const long long* __restrict__ plan, int count,
+                                                   double gamma, Gate gt)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count || !gate_open(gt)) {
+        return;
+    }
+    const long long dst = plan[3 * k];
+    const long long src = plan[3 * k + 1];
+    const int axis = static_cast<int>(plan[3 * k + 2]);
+    double W[kNC];
+    if (src >= 0) {
+        double S[kNC];
+#pragma unroll
+        for (int c = 0; c < kNC; ++c) {
+            S[c] = static_cast<double>(U[c * g.plane + src]);
+        }
+        // to_primitive: include/mhd/mhd_physics.hpp:23-39 (divides by rho)
+        W[0] = S[0];
+        W[1] = S[1] / S[0];
+        W[2] = S[2] / S[0];
+        W[3] = S[3] / S[0];
+        W[5] = S[5];
+        W[6] = S[6];
+        W[7] = S[7];
+        W[8] = S[8];
+        const double e_kin = 0.5 * (S[1] * S[1] + S[2] * S[2] + S[3] * S[3]) / S[0];
+        const double e_mag = 0.5 * (S[5] * S[5] + S[6] * S[6] + S[7] * S[7]);
+        W[4] = (gamma - 1.0) * (S[4] - e_kin - e_mag);
+        if (axis == 0) {
+            W[1] = -W[1];
+        } else {
+            W[2] = -W[2];
+        }
+        if (W[0] < 1e-12) {
+            W[0] = 1e-12;
+        }
+        if (W[4] < 1e-12) {
+            W[4] = 1e-12;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < kNC; ++c) {
+            W[c] = 0.0;
+        }
+        W[0] = 1.0;
+        W[4] = 1.0;
+    }
+    // to_conservative: include/mhd/mhd_physics.hpp:41-59
+    double O[kNC];
+    O[0] = W[0];
+    O[1] = W[0] * W[1];
+    O[2] = W[0] * W[2];
+    O[3] = W[0] * W[3];
+    O[5] = W[5];
+    O[6] = W[6];
+    O[7] = W[7];
+    O[8] = W[8];
+    const double ek = 0.5 * W[0] * (W[1] * W[1] + W[2] * W[2] + W[3] * W[3]);
+    const double em = 0.5 * (W[5] * W[5] + W[6] * W[6] + W[7] * W[7]);
+    O[4] = W[4] / (gamma - 1.0) + ek + em;
+#pragma unroll
+    for (int c = 0; c < kNC; ++c) {
+        U[c * g.plane + dst] = static_cast<R>(O[c]);
+    }
+}
+
+// ---- fused RK stage ------------------------------------------------------------------------
+//
+// One wave owns a strip of 60 interior columns and marches rows [r0, r1). Lane l holds global
+// column gi = ng + 60 s - 2 + l; lanes 2..61 own cells, lanes 2..62 compute the x-face on the
+// left of their column, lanes 0, 1, 63 only supply neighbours. Per row:
+//   * the new row c+1 is loaded (coalesced, 64 consecutive cells per component) and turned into
+//     primitives once;
+//   * y: the slope of row c comes from a three-row register window, the face c-1/2 from the
+//     carried left state, and the previous y-flux is carried, so each y-face is built once;
+//   * x (row c-1): neighbours arrive by lane shuffles, each cell's slope is built once, the
+//     left face state is shuffled from lane l-1 and the right flux from lane l+1.
+// Ghost cells are either read from memory (after k_ghost_fill) or, with REMAP, addressed at
+// their interior source directly, which equals the x-then-y fill.
+struct StageBuffers {
+    const void* src;  // stage input state (U in stage 1, U* in stage 2)
+    const void* u0;   // stage 2: U at the start of the step (aliases dst); stage 1: unused
+    void* dst;        // stage 1: U*; stage 2: U (in place)
+    const uint8_t* mask;
+    Ctrl* ctrl;
+    double* dt_log;
+    int dt_cap;
+};
+
+template <int STAGE, int LIM, int NI, int REMAP, typename R>
+__global__ __launch_bounds__(kBlock) void k_stage(StageBuffers b, Geom g, Physics ph, Tiling tl,
+                                                  int parity)
+{
+    const StepState st = step_state(b.ctrl, ph, parity);
+    const bool lead = blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0;
+    if (STAGE == 1 && lead) {
+        b.ctrl->chmax[parity ^ 1] = 0ull;
+        if (!st.active && ctrl_pending(b.ctrl, ph, parity)) {
+            b.ctrl->bad_dt = 1;
+        }
+    }
+    if (STAGE == 2 && lead) {
+        const int q = parity ^ 1;
+        b.ctrl->t[q] = st.active ? st.t + st.dt : st.t;
+        b.ctrl->steps[q] = st.active ? st.steps + 1 : st.steps;
+        b.ctrl->ch_last[q] = st.active ? st.c_h : b.ctrl->ch_last[parity];
+        if (st.active && b.dt_log != nullptr && st.steps < b.dt_cap) {
+            b.dt_log[st.steps] = st.dt;
+        }
+    }
+    if (!st.active) {
+        return;
+    }
+    // block (strip, segment group): its four waves march four consecutive row segments
+    const int lane = threadIdx.x & (kWave - 1);
+    const int strip = blockIdx.x;
+    const int seg = blockIdx.y * kWavesPerBlock + (threadIdx.x >> 6);
+    if (strip >= tl.strips || seg >= tl.segs) {
+        return;
+    }
+    const int r0 = g.ng + seg * tl.seg_rows;
+    const int r1 = min(r0 + tl.seg_rows, g.ng + g.ny);
+
+    const R* __restrict__ src = static_cast<const R*>(b.src);
+    const R* u0 = static_cast<const R*>(b.u0);
+    R* dst = static_cast<R*>(b.dst);
+    const long long P = g.plane;
+
+    const int gi = g.ng + strip * kStripCells - 2 + lane;
+    const int gic = min(gi, g.nxt - 1);
+    const int col = REMAP ? ghost_source(gic, g.nx, g.ng, g.bc_x) : gic;
+    const bool cell_lane = lane >= 2 && lane < 2 + kStripCells && gi < g.ng + g.nx;
+
+    const R gamma = static_cast<R>(ph.gamma);
+    const R gm1 = static_cast<R>(ph.gm1);
+    const R inv_gm1 = static_cast<R>(ph.inv_gm1);
+    const R c_h = static_cast<R>(st.c_h);
+    const R inv_dx = static_cast<R>(ph.inv_dx);
+    const R inv_dy = static_cast<R>(ph.inv_dy);
+    const R inv_2dx = static_cast<R>(ph.inv_2dx);
+    const R inv_2dy = static_cast<R>(ph.inv_2dy);
+    const R half(0.5);
+    // include/mhd/rhs.hpp:173-174
+    const R psi_damp = (st.c_h > 0.0 && ph.glm_alpha > 0.0)
+                           ? static_cast<R>(st.c_h * ph.glm_alpha / ph.h)
+                           : R(0);
+    const R dt = static_cast<R>(st.dt);
+    const R half_dt = static_cast<R>(0.5 * st.dt);
+
+    auto load_row = [&](int q, R (&u)[kNC]) {
+        const int qr = REMAP ? ghost_source(q, g.ny, g.ng, g.bc_y) : q;
+        const long long base = static_cast<long long>(qr) * g.nxt + col;
+#pragma unroll
+        for (int c = 0; c < kNC; ++c) {
+            u[c] = src[c * P + base];
+        }
+    };
+
+    R Wa[kNC], Wb[kNC], Wn[kNC], Uraw[kNC];
+    R WLy[kNC], Gp[kNC];
+    R Jp[3] = {R(0), R(0), R(0)};
+    R local_speed = R(0);
+
+    load_row(r0 - 2, Uraw);
+    to_prim(Uraw, gm1, Wa);
+    load_row(r0 - 1, Uraw);
+    to_prim(Uraw, gm1, Wb);
+    load_row(r0, Uraw);
+
+    for (int c = r0 - 1; c <= r1; ++c) {
+        to_prim(Uraw, gm1, Wn);  // row c + 1
+        if (c + 2 <= r1 + 1) {
+            load_row(c + 2, Uraw);
+        }
+        // y slope of row c and the two face states it produces
+        R WRy[kNC], WLn[kNC];
+#pragma unroll
+        for (int k = 0; k < kNC; ++k) {
+            const R d = limit_slope<LIM>(Wb[k] - Wa[k], Wn[k] - Wb[k]);
+            WRy[k] = Wb[k] - half * d;
+            WLn[k] = Wb[k] + half * d;
+        }
+        R Jc[3] = {R(0), R(0), R(0)};
+        if (NI) {
+            // J = curl B at (gi, c): include/mhd/nonideal.hpp:30-60
+            const R byL = __shfl_up(Wb[BY], 1);
+            const R byR = __shfl_down(Wb[BY], 1);
+            const R bzL = __shfl_up(Wb[BZ], 1);
+            const R bzR = __shfl_down(Wb[BZ], 1);
+            const R dbz_dy = (Wn[BZ] - Wa[BZ]) * inv_2dy;
+            const R dbz_dx = (bzR - bzL) * inv_2dx;
+            const R dby_dx = (byR - byL) * inv_2dx;
+            const R dbx_dy = (Wn[BX] - Wa[BX]) * inv_2dy;
+            Jc[0] = dbz_dy;
+            Jc[1] = -dbz_dx;
+            Jc[2] = dby_dx - dbx_dy;
+        }
+        if (c >= r0) {
+            // y-face c - 1/2 between rows c-1 (Wa) and c (Wb)
+            R G[kNC];
+            hll_face<1>(WLy, WRy, Wa, Wb, gamma, inv_gm1, c_h, G);
+            if (NI) {
+                // include/mhd/rhs.hpp:96-114
+                R Jf[3], Bf[3], E[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    Jf[k] = half * (Jp[k] + Jc[k]);
+                    Bf[k] = half * (Wa[BX + k] + Wb[BX + k]);
+                }
+                nonideal_e(Jf, Bf, ph.ni, E);
+                G[BX] += E[2];
+                G[BZ] += -E[0];
+                G[PR] += Bf[2] * E[0] - Bf[0] * E[2];
+            }
+            if (c >= r0 + 1) {
+                // x-direction of row c - 1 (primitives Wa), then assemble and update it
+                const int rr = c - 1;
+                R Wl[kNC], WRx[kNC], WLx[kNC];
+#pragma unroll
+                for (int k = 0; k < kNC; ++k) {
+                    Wl[k] = __shfl_up(Wa[k], 1);
+                    const R wr = __shfl_down(Wa[k], 1);
+                    const R d = limit_slope<LIM>(Wa[k] - Wl[k], wr - Wa[k]);
+                    WRx[k] = Wa[k] - half * d;
+                    WLx[k] = __shfl_up(Wa[k] + half * d, 1);
+                }
+                R F[kNC];
+                hll_face<0>(WLx, WRx, Wl, Wa, gamma, inv_gm1, c_h, F);
+                if (NI) {
+                    // include/mhd/rhs.hpp:74-93
+                    R Jf[3], Bf[3], E[3];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        Jf[k] = half * (__shfl_up(Jp[k], 1) + Jp[k]);
+                        Bf[k] = half * (Wl[BX + k] + Wa[BX + k]);
+                    }
+                    nonideal_e(Jf, Bf, ph.ni, E);
+                    F[BY] += -E[2];
+                    F[BZ] += E[1];
+                    F[PR] += Bf[1] * E[2] - Bf[2] * E[1];
+                }
+                R L[kNC];
+#pragma unroll
+                for (int k = 0; k < kNC; ++k) {
+                    const R Fr = __shfl_down(F[k], 1);
+                    // include/mhd/rhs.hpp:116-123
+                    L[k] = -(Fr - F[k]) * inv_dx - (G[k] - Gp[k]) * inv_dy;
+                }
+                if (psi_damp != R(0)) {
+                    L[PSI] -= psi_damp * Wa[PSI];
+                }
+                if (cell_lane) {
+                    if (b.mask != nullptr &&
+                        b.mask[static_cast<long long>(rr - g.ng) * g.nx + (gi - g.ng)] == 0) {
+#pragma unroll
+                        for (int k = 0; k < kNC; ++k) {
+                            L[k] = R(0);
+                        }
+                    }
+                    const long long cell = static_cast<long long>(rr) * g.nxt + gi;
+                    if (STAGE == 1) {
+                        // include/mhd/field_ops.hpp:47-53
+#pragma unroll
+                        for (int k = 0; k < kNC; ++k) {
+                            dst[k * P + cell] = src[k * P + cell] + dt * L[k];
+                        }
+                    } else {
+                        // include/mhd/field_ops.hpp:55-63
+                        R un[kNC];
+#pragma unroll
+                        for (int k = 0; k < kNC; ++k) {
+                            un[k] = half * (u0[k * P + cell] + src[k * P + cell]) + half_dt * L[k];
+                        }
+#pragma unroll
+                        for (int k = 0; k < kNC; ++k) {
+                            dst[k * P + cell] = un[k];
+                        }
+                        const bool fluid =
+                            b.mask == nullptr ||
+                            b.mask[static_cast<long long>(rr - g.ng) * g.nx + (gi - g.ng)] != 0;
+                        if (fluid) {
+                            local_speed = vmax(local_speed, cell_max_speed(un, gamma, gm1));
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kNC; ++k) {
+                Gp[k] = G[k];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kNC; ++k) {
+            WLy[k] = WLn[k];
+            Wa[k] = Wb[k];
+            Wb[k] = Wn[k];
+        }
+        if (NI) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                Jp[k] = Jc[k];
+            }
+        }
+    }
+    if (STAGE == 2) {
+        const R m = wave_max(local_speed);
+        if (lane == 0) {
+            atomic_max_speed(&b.ctrl->chmax[parity ^ 1], static_cast<double>(m));
+        }
+    }
+}
+
+}  // namespace mhdhip

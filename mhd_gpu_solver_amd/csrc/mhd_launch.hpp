@@ -1,0 +1,34 @@
+// Host-side launchers of the step kernels, one explicit instantiation per precision:
+// mhd_fp64.hip (double, compiled with -ffp-contract=off) and mhd_fp32.hip (float, contraction on).
+#pragma once
+
+#include "mhd_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+namespace mhdhip {
+
+// Wave/segment tiling of the fused stage kernel for an nx x ny interior.
+Tiling make_tiling(int nx, int ny, int seg_rows);
+int default_seg_rows(int nx, int ny);
+
+template <typename R>
+hipError_t launch_ctrl_init(Ctrl* ctrl, hipStream_t s);
+
+template <typename R>
+hipError_t launch_cfl(const R* U, const Geom& g, const Physics& ph, const uint8_t* mask, Ctrl* ctrl,
+                      hipStream_t s);
+
+template <typename R>
+hipError_t launch_ghost_fill(R* U, const Geom& g, const Gate& gt, hipStream_t s);
+
+template <typename R>
+hipError_t launch_wall_fill(R* U, const Geom& g, const long long* plan, int count, double gamma,
+                            const Gate& gt, hipStream_t s);
+
+template <typename R>
+hipError_t launch_stage(int stage, int limiter, bool nonideal, bool remap, const StageBuffers& b,
+                        const Geom& g, const Physics& ph, const Tiling& tl, int parity,
+                        hipStream_t s);
+
+}  // namespace mhdhip

@@ -1,0 +1,111 @@
+// Launcher definitions shared by the per-precision translation units.
+#pragma once
+
+#include "mhd_launch.hpp"
+
+#include <algorithm>
+
+namespace mhdhip {
+
+namespace detail {
+
+inline dim3 stage_grid(const Tiling& tl)
+{
+    // blockIdx.x -> strip, blockIdx.y * waves + wave -> row segment
+    return dim3(static_cast<unsigned>(tl.strips),
+                static_cast<unsigned>((tl.segs + kWavesPerBlock - 1) / kWavesPerBlock), 1);
+}
+
+inline unsigned flat_blocks(long long work, int block, int cap)
+{
+    const long long b = (work + block - 1) / block;
+    return static_cast<unsigned>(std::max<long long>(1, std::min<long long>(b, cap)));
+}
+
+template <int STAGE, int LIM, int NI, int REMAP, typename R>
+hipError_t launch_one(const StageBuffers& b, const Geom& g, const Physics& ph, const Tiling& tl,
+                      int parity, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_stage<STAGE, LIM, NI, REMAP, R>), stage_grid(tl), dim3(kBlock), 0, s, b,
+                       g, ph, tl, parity);
+    return hipGetLastError();
+}
+
+template <int STAGE, int LIM, typename R>
+hipError_t launch_ni_remap(bool ni, bool remap, const StageBuffers& b, const Geom& g,
+                           const Physics& ph, const Tiling& tl, int parity, hipStream_t s)
+{
+    if (ni) {
+        return remap ? launch_one<STAGE, LIM, 1, 1, R>(b, g, ph, tl, parity, s)
+                     : launch_one<STAGE, LIM, 1, 0, R>(b, g, ph, tl, parity, s);
+    }
+    return remap ? launch_one<STAGE, LIM, 0, 1, R>(b, g, ph, tl, parity, s)
+                 : launch_one<STAGE, LIM, 0, 0, R>(b, g, ph, tl, parity, s);
+}
+
+}  // namespace detail
+
+template <typename R>
+hipError_t launch_ctrl_init(Ctrl* ctrl, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_ctrl_init<R>), dim3(1), dim3(64), 0, s, ctrl);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_cfl(const R* U, const Geom& g, const Physics& ph, const uint8_t* mask, Ctrl* ctrl,
+                      hipStream_t s)
+{
+    const long long ncell = static_cast<long long>(g.nx) * g.ny;
+    hipLaunchKernelGGL((k_cfl<R>), dim3(detail::flat_blocks(ncell, 256, 4096)), dim3(256), 0, s, U,
+                       g, ph, mask, ctrl);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_ghost_fill(R* U, const Geom& g, const Gate& gt, hipStream_t s)
+{
+    const long long total = 2LL * g.ng * g.nxt + 2LL * g.ng * g.ny;
+    hipLaunchKernelGGL((k_ghost_fill<R>), dim3(detail::flat_blocks(total, 256, 2048)), dim3(256), 0,
+                       s, U, g, gt);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_wall_fill(R* U, const Geom& g, const long long* plan, int count, double gamma,
+                            const Gate& gt, hipStream_t s)
+{
+    if (count <= 0) {
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL((k_wall_fill<R>), dim3(detail::flat_blocks(count, 256, 1 << 20)), dim3(256),
+                       0, s, U, g, plan, count, gamma, gt);
+    return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_stage(int stage, int limiter, bool nonideal, bool remap, const StageBuffers& b,
+                        const Geom& g, const Physics& ph, const Tiling& tl, int parity,
+                        hipStream_t s)
+{
+    if (stage == 1) {
+        return limiter == kMinmod
+                   ? detail::launch_ni_remap<1, kMinmod, R>(nonideal, remap, b, g, ph, tl, parity, s)
+                   : detail::launch_ni_remap<1, kMC, R>(nonideal, remap, b, g, ph, tl, parity, s);
+    }
+    return limiter == kMinmod
+               ? detail::launch_ni_remap<2, kMinmod, R>(nonideal, remap, b, g, ph, tl, parity, s)
+               : detail::launch_ni_remap<2, kMC, R>(nonideal, remap, b, g, ph, tl, parity, s);
+}
+
+#define MHDHIP_INSTANTIATE(R)                                                                    \
+    template hipError_t launch_ctrl_init<R>(Ctrl*, hipStream_t);                                 \
+    template hipError_t launch_cfl<R>(const R*, const Geom&, const Physics&, const uint8_t*,     \
+                                      Ctrl*, hipStream_t);                                       \
+    template hipError_t launch_ghost_fill<R>(R*, const Geom&, const Gate&, hipStream_t);         \
+    template hipError_t launch_wall_fill<R>(R*, const Geom&, const long long*, int, double,      \
+                                            const Gate&, hipStream_t);                           \
+    template hipError_t launch_stage<R>(int, int, bool, bool, const StageBuffers&, const Geom&,  \
+                                        const Physics&, const Tiling&, int, hipStream_t);
+
+}  // namespace mhdhip

@@ -1,0 +1,326 @@
+// Device point physics of the GLM-MHD MUSCL + HLL scheme, written for CDNA (gfx950).
+//
+// Every routine keeps the reference CPU kernels' operation order (include/mhd/kernels/
+// muscl_hll.hpp:33-239, include/mhd/kernels/cpu_rows.hpp:130-155, include/mhd/nonideal.hpp:63-102)
+// so that the fp64 instantiation, compiled with -ffp-contract=off, rounds exactly like the CPU path.
+//
+// Values are kept in the physical component order (rho, vx, vy, vz, p, bx, by, bz, psi).
+// A face along direction DIR (0 = x, 1 = y) reads its normal/tangential slots through the
+// compile-time indices of FaceDir<DIR>, which replaces the reference's load/store rotation.
+//
+// Precision policy (template parameter R):
+//   double: everything in fp64.
+//   float : bulk arithmetic in fp32; pressure recovery from total energy and the fast-speed
+//           discriminant are evaluated in fp64 (MixedOps<float>), because both subtract nearly
+//           equal quantities at low plasma beta / strong field.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace mhdhip {
+
+constexpr int kNC = 9;
+enum Comp { RHO = 0, VX = 1, VY = 2, VZ = 3, PR = 4, BX = 5, BY = 6, BZ = 7, PSI = 8 };
+enum Limiter { kMinmod = 0, kMC = 1 };
+
+template <int DIR>
+struct FaceDir {
+    static constexpr int N = DIR ? VY : VX;   // normal velocity / momentum slot
+    static constexpr int T = DIR ? VX : VY;   // tangential velocity / momentum slot
+    static constexpr int BN = DIR ? BY : BX;  // normal field slot
+    static constexpr int BT = DIR ? BX : BY;  // tangential field slot
+};
+
+template <typename R>
+struct Lit;
+template <>
+struct Lit<double> {
+    static __device__ __forceinline__ double inf() { return __builtin_huge_val(); }
+    static constexpr double kDegenerate = 1.0e-14;
+};
+template <>
+struct Lit<float> {
+    static __device__ __forceinline__ float inf() { return __builtin_huge_valf(); }
+    static constexpr float kDegenerate = 1.0e-14f;
+};
+
+// min/max with the reference's tie and NaN rule (include/mhd/simd.hpp:40-47)
+template <typename R>
+__device__ __forceinline__ R vmin(R a, R b) { return (b < a) ? b : a; }
+template <typename R>
+__device__ __forceinline__ R vmax(R a, R b) { return (a < b) ? b : a; }
+
+__device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
+__device__ __forceinline__ float dsqrt(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double dabs(double x) { return __builtin_fabs(x); }
+__device__ __forceinline__ float dabs(float x) { return __builtin_fabsf(x); }
+
+// ---- limiters: muscl_hll.hpp:33-57 ------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ R minmod(R a, R b)
+{
+    const R zero(0);
+    return (a * b <= zero) ? zero : ((dabs(a) < dabs(b)) ? a : b);
+}
+
+template <int LIM, typename R>
+__device__ __forceinline__ R limit_slope(R a, R b)
+{
+    if constexpr (LIM == kMinmod) {
+        return minmod(a, b);
+    } else {
+        const R half(0.5);
+        const R two(2);
+        return minmod(half * (a + b), minmod(two * a, two * b));
+    }
+}
+
+// ---- mixed-precision hooks ----------------------------------------------------------------
+template <typename R>
+struct MixedOps;
+
+template <>
+struct MixedOps<double> {
+    // cf^2 discriminant max(sum^2 - 4 cs2 van2, 0): muscl_hll.hpp:84
+    static __device__ __forceinline__ double disc(double sum, double cs2, double van2)
+    {
+        return vmax(sum * sum - 4.0 * cs2 * van2, 0.0);
+    }
+    // p = (gamma-1) (E - ekin - emag) with ekin = 0.5 |m|^2 inv_rho: cpu_rows.hpp:146-153
+    static __device__ __forceinline__ double pressure(double rho, double inv_rho, double mx,
+                                                      double my, double mz, double e, double bx,
+                                                      double by, double bz, double gm1)
+    {
+        (void)rho;
+        const double ekin = 0.5 * (mx * mx + my * my + mz * mz) * inv_rho;
+        const double emag = 0.5 * (bx * bx + by * by + bz * bz);
+        return gm1 * (e - ekin - emag);
+    }
+};
+
+template <>
+struct MixedOps<float> {
+    static __device__ __forceinline__ float disc(float sum, float cs2, float van2)
+    {
+        // fp32 x fp32 products are exact in fp64; one fp64 rounding for the difference.
+        const double s = static_cast<double>(sum);
+        const double d = s * s - 4.0 * (static_cast<double>(cs2) * static_cast<double>(van2));
+        return static_cast<float>(d > 0.0 ? d : 0.0);
+    }
+    static __device__ __forceinline__ float pressure(float rho, float inv_rho, float mx, float my,
+                                                     float mz, float e, float bx, float by,
+                                                     float bz, float gm1)
+    {
+        (void)inv_rho;
+        const double dmx = mx, dmy = my, dmz = mz, dbx = bx, dby = by, dbz = bz;
+        const double ekin = 0.5 * (dmx * dmx + dmy * dmy + dmz * dmz) / static_cast<double>(rho);
+        const double emag = 0.5 * (dbx * dbx + dby * dby + dbz * dbz);
+        return static_cast<float>(static_cast<double>(gm1) *
+                                  (static_cast<double>(e) - ekin - emag));
+    }
+};
+
+// ---- fast magnetosonic speed along DIR: muscl_hll.hpp:70-87 --------------------------------
+template <int DIR, typename R>
+__device__ __forceinline__ R fast_speed(const R (&W)[kNC], R gamma)
+{
+    using D = FaceDir<DIR>;
+    const R half(0.5);
+    const R inv_rho = R(1) / W[RHO];
+    const R cs2 = gamma * W[PR] * inv_rho;
+    const R bn2 = W[D::BN] * W[D::BN];
+    const R b2 = bn2 + W[D::BT] * W[D::BT] + W[BZ] * W[BZ];
+    const R va2 = b2 * inv_rho;
+    const R van2 = bn2 * inv_rho;
+    const R sum = cs2 + va2;
+    const R disc = MixedOps<R>::disc(sum, cs2, van2);
+    const R cf2 = half * (sum + dsqrt(disc));
+    return dsqrt(vmax(cf2, R(0)));
+}
+
+// ---- conserved state and GLM-MHD flux along DIR: muscl_hll.hpp:89-124 ---------------------
+template <int DIR, typename R>
+__device__ __forceinline__ void cons_flux(const R (&W)[kNC], R inv_gm1, R ch2, R (&U)[kNC],
+                                          R (&F)[kNC])
+{
+    using D = FaceDir<DIR>;
+    const R half(0.5);
+    const R rho = W[RHO], vn = W[D::N], vt = W[D::T], vz = W[VZ], p = W[PR];
+    const R bn = W[D::BN], bt = W[D::BT], bz = W[BZ], psi = W[PSI];
+    const R mn = rho * vn;
+    const R mt = rho * vt;
+    const R mz = rho * vz;
+    const R v2 = vn * vn + vt * vt + vz * vz;
+    const R b2 = bn * bn + bt * bt + bz * bz;
+    const R pmag = half * b2;
+    const R energy = p * inv_gm1 + half * rho * v2 + pmag;
+    const R pstar = p + pmag;
+    const R vb = vn * bn + vt * bt + vz * bz;
+
+    U[RHO] = rho;
+    U[D::N] = mn;
+    U[D::T] = mt;
+    U[VZ] = mz;
+    U[PR] = energy;
+    U[D::BN] = bn;
+    U[D::BT] = bt;
+    U[BZ] = bz;
+    U[PSI] = psi;
+
+    F[RHO] = mn;
+    F[D::N] = mn * vn + pstar - bn * bn;
+    F[D::T] = mn * vt - bn * bt;
+    F[VZ] = mn * vz - bn * bz;
+    F[PR] = (energy + pstar) * vn - bn * vb + psi * bn;
+    F[D::BN] = psi;
+    F[D::BT] = vn * bt - vt * bn;
+    F[BZ] = vn * bz - vz * bn;
+    F[PSI] = ch2 * bn;
+}
+
+// ---- HLL + GLM flux from reconstructed face states: muscl_hll.hpp:145-209 ----------------
+// WL/WR are the MUSCL states; Wm/W0 are the cells left/right of the face, used when either
+// reconstructed state is non-positive or non-finite (both sides then revert to first order).
+template <int DIR, typename R>
+__device__ __forceinline__ void hll_face(R (&WL)[kNC], R (&WR)[kNC], const R (&Wm)[kNC],
+                                         const R (&W0)[kNC], R gamma, R inv_gm1, R c_h,
+                                         R (&F)[kNC])
+{
+    using D = FaceDir<DIR>;
+    const R zero(0);
+    const R inf = Lit<R>::inf();
+    const bool ok = (WL[RHO] > zero && WL[PR] > zero && WL[RHO] < inf && WL[PR] < inf) &&
+                    (WR[RHO] > zero && WR[PR] > zero && WR[RHO] < inf && WR[PR] < inf);
+#pragma unroll
+    for (int c = 0; c < kNC; ++c) {
+        WL[c] = ok ? WL[c] : Wm[c];
+        WR[c] = ok ? WR[c] : W0[c];
+    }
+    const R cfL = fast_speed<DIR>(WL, gamma);
+    const R cfR = fast_speed<DIR>(WR, gamma);
+    const R sL = vmin(vmin(WL[D::N] - cfL, WR[D::N] - cfR), -c_h);
+    const R sR = vmax(vmax(WL[D::N] + cfL, WR[D::N] + cfR), c_h);
+    const R ch2 = c_h * c_h;
+
+    R UL[kNC], FL[kNC], UR[kNC], FR[kNC];
+    cons_flux<DIR>(WL, inv_gm1, ch2, UL, FL);
+    cons_flux<DIR>(WR, inv_gm1, ch2, UR, FR);
+
+    const R denom = sR - sL;
+    const R inv_denom = R(1) / denom;
+    const R sLsR = sL * sR;
+    const R half(0.5);
+    const bool use_left = sL >= zero;
+    const bool use_right = sR <= zero;
+    const bool degenerate = dabs(denom) < Lit<R>::kDegenerate;
+#pragma unroll
+    for (int c = 0; c < kNC; ++c) {
+        const R hll = (sR * FL[c] - sL * FR[c] + sLsR * (UR[c] - UL[c])) * inv_denom;
+        const R mid = degenerate ? half * (FL[c] + FR[c]) : hll;
+        F[c] = use_left ? FL[c] : (use_right ? FR[c] : mid);
+    }
+}
+
+// ---- conserved -> primitive: cpu_rows.hpp:130-155 ------------------------------------------
+template <typename R>
+__device__ __forceinline__ void to_prim(const R (&U)[kNC], R gm1, R (&W)[kNC])
+{
+    const R inv_rho = R(1) / U[RHO];
+    W[RHO] = U[RHO];
+    W[VX] = U[1] * inv_rho;
+    W[VY] = U[2] * inv_rho;
+    W[VZ] = U[3] * inv_rho;
+    W[PR] = MixedOps<R>::pressure(U[RHO], inv_rho, U[1], U[2], U[3], U[4], U[5], U[6], U[7], gm1);
+    W[BX] = U[5];
+    W[BY] = U[6];
+    W[BZ] = U[7];
+    W[PSI] = U[8];
+}
+
+// ---- CFL signal speed max(|vx|+cf_x, |vy|+cf_y) of one cell: muscl_hll.hpp:211-239 ---------
+template <typename R>
+__device__ __forceinline__ R cell_max_speed(const R (&U)[kNC], R gamma, R gm1)
+{
+    const R inv_rho = R(1) / U[RHO];
+    R W[kNC];
+    W[RHO] = U[RHO];
+    W[VX] = U[1] * inv_rho;
+    W[VY] = U[2] * inv_rho;
+    W[VZ] = U[3] * inv_rho;
+    W[PR] = MixedOps<R>::pressure(U[RHO], inv_rho, U[1], U[2], U[3], U[4], U[5], U[6], U[7], gm1);
+    W[BX] = U[5];
+    W[BY] = U[6];
+    W[BZ] = U[7];
+    W[PSI] = R(0);
+    // x: (n, t) = (x, y); y: (n, t) = (y, x) -- the y sum order is by^2 + bx^2 + bz^2
+    const R sx = dabs(W[VX]) + fast_speed<0>(W, gamma);
+    const R sy = dabs(W[VY]) + fast_speed<1>(W, gamma);
+    return vmax(sx, sy);
+}
+
+// ---- non-ideal electric field: nonideal.hpp:63-102 -----------------------------------------
+struct NonIdeal {
+    int ohmic, hall, ambipolar;
+    double eta_ohm, eta_hall, eta_ambipolar;
+};
+
+template <typename R>
+__device__ __forceinline__ void nonideal_e(const R (&J)[3], const R (&B)[3], const NonIdeal& cfg,
+                                           R (&E)[3])
+{
+    E[0] = R(0);
+    E[1] = R(0);
+    E[2] = R(0);
+    if (cfg.ohmic) {
+        const R eta = static_cast<R>(cfg.eta_ohm);
+        E[0] += eta * J[0];
+        E[1] += eta * J[1];
+        E[2] += eta * J[2];
+    }
+    if (!(cfg.hall || cfg.ambipolar)) {
+        return;
+    }
+    const R B2 = B[0] * B[0] + B[1] * B[1] + B[2] * B[2];
+    if (B2 < R(1.0e-14)) {
+        return;
+    }
+    const R jb0 = J[1] * B[2] - J[2] * B[1];
+    const R jb1 = J[2] * B[0] - J[0] * B[2];
+    const R jb2 = J[0] * B[1] - J[1] * B[0];
+    if (cfg.hall) {
+        const R eta = static_cast<R>(cfg.eta_hall);
+        const R invB = R(1) / dsqrt(B2);
+        E[0] += eta * jb0 * invB;
+        E[1] += eta * jb1 * invB;
+        E[2] += eta * jb2 * invB;
+    }
+    if (cfg.ambipolar) {
+        const R eta = static_cast<R>(cfg.eta_ambipolar);
+        const R a0 = jb1 * B[2] - jb2 * B[1];
+        const R a1 = jb2 * B[0] - jb0 * B[2];
+        const R a2 = jb0 * B[1] - jb1 * B[0];
+        const R invB2 = R(1) / B2;
+        E[0] += eta * a0 * invB2;
+        E[1] += eta * a1 * invB2;
+        E[2] += eta * a2 * invB2;
+    }
+}
+
+// ---- ghost index maps: boundary_conditions.hpp:26-30 (periodic), :44-46 (outflow) ---------
+// Maps a global index g of an axis with n interior cells and ng ghosts to the interior cell
+// whose value the ghost holds after the x-then-y ghost fill. Interior indices map to themselves.
+__device__ __host__ __forceinline__ int ghost_source(int g, int n, int ng, int bc)
+{
+    if (g >= ng && g < ng + n) {
+        return g;
+    }
+    if (bc == 0) {  // periodic: ng + ((g - ng) mod n)
+        const int off = g - ng;
+        return ng + ((off % n) + n) % n;
+    }
+    return g < ng ? ng : ng + n - 1;  // outflow: nearest interior cell
+}
+
+}  // namespace mhdhip

@@ -1,0 +1,792 @@
+// Single-run device solver and the C ABI of include/mhd_hip.h.
+//
+// One mhd_hip_solver owns the device state of one grid: U, U* (9 planes each, fp64 or fp32),
+// the control block, the interior mask and the wall-fill plan. A solve runs on the solver's own
+// stream: the first CFL reduction, then batches of steps replayed from a captured hipGraph
+// (kSteps steps per replay, parity baked in), checking the device control block once per
+// replay while the next replay is already queued. Surplus steps after t_end are no-ops inside
+// the kernels, so the host never needs the step count in advance.
+#include "mhd_hip.h"
+#include "mhd_host.hpp"
+#include "mhd_launch.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct InvalidArgument : std::invalid_argument {
+    using std::invalid_argument::invalid_argument;
+};
+
+struct NoDevice : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) {
+        throw std::runtime_error(std::string("HIP error in ") + what + ": " +
+                                 hipGetErrorString(e));
+    }
+}
+
+template <class F>
+int guarded(F&& f)
+{
+    try {
+        f();
+        g_last_error.clear();
+        return MHD_HIP_OK;
+    } catch (const NoDevice& e) {
+        g_last_error = e.what();
+        return MHD_HIP_ENODEV;
+    } catch (const std::invalid_argument& e) {
+        g_last_error = e.what();
+        return MHD_HIP_EINVAL;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return MHD_HIP_ERUNTIME;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return MHD_HIP_ERUNTIME;
+    }
+}
+
+bool device_is_gfx950(int dev)
+{
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        return false;
+    }
+    return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
+}
+
+}  // namespace
+
+using mhdhip::Ctrl;
+using mhdhip::Gate;
+using mhdhip::Geom;
+using mhdhip::Physics;
+using mhdhip::StageBuffers;
+using mhdhip::Tiling;
+
+struct mhd_hip_solver {
+    static constexpr int kSteps = 8;  // steps per captured graph (even: parity 0,1,...)
+
+    int device = 0;
+    mhd_hip_grid grid{};
+    int precision = MHD_HIP_FP64;
+    Geom geom{};
+    double dx = 0.0, dy = 0.0;
+    size_t elem = 8;
+
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    void* U = nullptr;
+    void* Us = nullptr;
+    Ctrl* ctrl = nullptr;
+    Ctrl* ctrl_host = nullptr;  // pinned
+    double* dt_log = nullptr;
+    int dt_cap = 0;
+    int dt_count = 0;
+
+    int bc_x = MHD_HIP_PERIODIC, bc_y = MHD_HIP_PERIODIC;
+    bool masked = false;
+    bool walls = true;
+    uint8_t* mask_dev = nullptr;
+    long long* plan_dev = nullptr;
+    int plan_count = 0;
+
+    int seg_rows = 0;  // 0 = heuristic
+    double last_seconds = 0.0;
+
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<unsigned char> graph_key;
+
+    ~mhd_hip_solver() { release(); }
+
+    void release()
+    {
+        if (device >= 0) {
+            hipSetDevice(device);
+        }
+        if (graph_exec) hipGraphExecDestroy(graph_exec);
+        if (U) hipFree(U);
+        if (Us) hipFree(Us);
+        if (ctrl) hipFree(ctrl);
+        if (ctrl_host) hipHostFree(ctrl_host);
+        if (dt_log) hipFree(dt_log);
+        if (mask_dev) hipFree(mask_dev);
+        if (plan_dev) hipFree(plan_dev);
+        if (ev_start) hipEventDestroy(ev_start);
+        if (ev_stop) hipEventDestroy(ev_stop);
+        if (stream) hipStreamDestroy(stream);
+        graph_exec = nullptr;
+        U = Us = nullptr;
+        ctrl = nullptr;
+        ctrl_host = nullptr;
+        dt_log = nullptr;
+        mask_dev = nullptr;
+        plan_dev = nullptr;
+        ev_start = ev_stop = nullptr;
+        stream = nullptr;
+    }
+
+    size_t plane() const { return static_cast<size_t>(geom.plane); }
+    size_t state_bytes() const { return 9 * plane() * elem; }
+
+    void init(const mhd_hip_grid& g, int prec)
+    {
+        if (g.ng < 2) {
+            throw InvalidArgument("MHDSolver requires ng >= 2");
+        }
+        if (g.nx < 1 || g.ny < 1) {
+            throw InvalidArgument("grid must have at least one interior cell per axis");
+        }
+        if (prec != MHD_HIP_FP64 && prec != MHD_HIP_FP32) {
+            throw InvalidArgument("precision must be MHD_HIP_FP64 or MHD_HIP_FP32");
+        }
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count < 1) {
+            throw NoDevice("no HIP device available");
+        }
+        check(hipGetDevice(&device), "hipGetDevice");
+        if (!device_is_gfx950(device)) {
+            throw NoDevice("current HIP device is not gfx950 (MI355X)");
+        }
+        grid = g;
+        precision = prec;
+        elem = prec == MHD_HIP_FP64 ? sizeof(double) : sizeof(float);
+        geom.nx = g.nx;
+        geom.ny = g.ny;
+        geom.ng = g.ng;
+        geom.nxt = g.nx + 2 * g.ng;
+        geom.nyt = g.ny + 2 * g.ng;
+        geom.plane = static_cast<long long>(geom.nxt) * geom.nyt;
+        geom.bc_x = MHD_HIP_PERIODIC;
+        geom.bc_y = MHD_HIP_PERIODIC;
+        // include/grid/grid.hpp:24-25
+        dx = (g.x_max - g.x_min) / static_cast<double>(g.nx);
+        dy = (g.y_max - g.y_min) / static_cast<double>(g.ny);
+
+        check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+        check(hipEventCreate(&ev_start), "hipEventCreate");
+        check(hipEventCreate(&ev_stop), "hipEventCreate");
+        check(hipMalloc(&U, state_bytes()), "hipMalloc(U)");
+        check(hipMalloc(&Us, state_bytes()), "hipMalloc(U*)");
+        check(hipMalloc(reinterpret_cast<void**>(&ctrl), sizeof(Ctrl)), "hipMalloc(ctrl)");
+        check(hipHostMalloc(reinterpret_cast<void**>(&ctrl_host), sizeof(Ctrl), 0),
+              "hipHostMalloc(ctrl)");
+        // zero both states so that never-uploaded ghosts are defined
+        check(hipMemsetAsync(U, 0, state_bytes(), stream), "hipMemset");
+        check(hipMemsetAsync(Us, 0, state_bytes(), stream), "hipMemset");
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    }
+
+    void configure(int bx, int by, const uint8_t* mask, size_t mask_len, int reflective)
+    {
+        auto valid = [](int b) { return b == MHD_HIP_PERIODIC || b == MHD_HIP_OUTFLOW; };
+        if (bx == MHD_HIP_REFLECTIVE || by == MHD_HIP_REFLECTIVE) {
+            // include/boundary_conditions/boundary_conditions.hpp:52-55
+            throw InvalidArgument("Reflective boundary conditions are not implemented yet");
+        }
+        if (!valid(bx) || !valid(by)) {
+            throw InvalidArgument("unknown boundary condition type");
+        }
+        hipSetDevice(device);
+        bc_x = bx;
+        bc_y = by;
+        geom.bc_x = bx;
+        geom.bc_y = by;
+        walls = reflective != 0;
+        masked = false;
+        plan_count = 0;
+        if (mask != nullptr) {
+            const size_t need = static_cast<size_t>(grid.nx) * grid.ny;
+            if (mask_len != need) {
+                throw InvalidArgument("FluidMask size does not match Grid2D interior");
+            }
+            masked = std::any_of(mask, mask + need, [](uint8_t v) { return v == 0; });
+        }
+        invalidate_graph();
+        if (!masked) {
+            return;
+        }
+        const size_t need = static_cast<size_t>(grid.nx) * grid.ny;
+        if (mask_dev == nullptr) {
+            check(hipMalloc(reinterpret_cast<void**>(&mask_dev), need), "hipMalloc(mask)");
+        }
+        check(hipMemcpy(mask_dev, mask, need, hipMemcpyHostToDevice), "hipMemcpy(mask)");
+        std::vector<long long> plan;
+        plan_count = static_cast<int>(mhdhip::wall_plan(grid, mask, plan));
+        if (plan_dev) {
+            hipFree(plan_dev);
+            plan_dev = nullptr;
+        }
+        if (plan_count > 0) {
+            check(hipMalloc(reinterpret_cast<void**>(&plan_dev), plan.size() * sizeof(long long)),
+                  "hipMalloc(plan)");
+            check(hipMemcpy(plan_dev, plan.data(), plan.size() * sizeof(long long),
+                            hipMemcpyHostToDevice),
+                  "hipMemcpy(plan)");
+        }
+    }
+
+    void upload(const double* const comps[9], size_t n)
+    {
+        if (n != plane()) {
+            throw InvalidArgument("upload: size does not match the full SoA layout");
+        }
+        hipSetDevice(device);
+        if (precision == MHD_HIP_FP64) {
+            for (int c = 0; c < 9; ++c) {
+                check(hipMemcpy(static_cast<double*>(U) + c * plane(), comps[c], n * sizeof(double),
+                                hipMemcpyHostToDevice),
+                      "hipMemcpy(upload)");
+            }
+        } else {
+            std::vector<float> stage(n);
+            for (int c = 0; c < 9; ++c) {
+                for (size_t k = 0; k < n; ++k) {
+                    stage[k] = static_cast<float>(comps[c][k]);
+                }
+                check(hipMemcpy(static_cast<float*>(U) + c * plane(), stage.data(),
+                                n * sizeof(float), hipMemcpyHostToDevice),
+                      "hipMemcpy(upload)");
+            }
+        }
+    }
+
+    void download(double* const comps[9], size_t n)
+    {
+        if (n != plane()) {
+            throw InvalidArgument("download: size does not match the full SoA layout");
+        }
+        hipSetDevice(device);
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        if (precision == MHD_HIP_FP64) {
+            for (int c = 0; c < 9; ++c) {
+                check(hipMemcpy(comps[c], static_cast<double*>(U) + c * plane(), n * sizeof(double),
+                                hipMemcpyDeviceToHost),
+                      "hipMemcpy(download)");
+            }
+        } else {
+            std::vector<float> stage(n);
+            for (int c = 0; c < 9; ++c) {
+                check(hipMemcpy(stage.data(), static_cast<float*>(U) + c * plane(),
+                                n * sizeof(float), hipMemcpyDeviceToHost),
+                      "hipMemcpy(download)");
+                for (size_t k = 0; k < n; ++k) {
+                    comps[c][k] = static_cast<double>(stage[k]);
+                }
+            }
+        }
+    }
+
+    Physics physics(const mhd_hip_params& p, double t_end, int max_steps) const
+    {
+        Physics ph{};
+        ph.gamma = p.gamma;
+        ph.gm1 = p.gamma - 1.0;
+        ph.inv_gm1 = 1.0 / (p.gamma - 1.0);  // include/mhd/kernels/cpu_rows.hpp:104
+        ph.cfl = p.cfl;
+        ph.inv_dx = 1.0 / dx;
+        ph.inv_dy = 1.0 / dy;
+        ph.inv_2dx = 0.5 / dx;  // include/mhd/nonideal.hpp:33-34
+        ph.inv_2dy = 0.5 / dy;
+        ph.glm_alpha = p.glm_alpha;
+        ph.h = (dy < dx) ? dy : dx;
+        // include/physics_config/mhd_config.hpp:97-110
+        double eta = 0.0;
+        auto mx = [](double a, double b) { return (a < b) ? b : a; };
+        if (p.ohmic) eta = mx(eta, std::fabs(p.eta_ohm));
+        if (p.hall) eta = mx(eta, std::fabs(p.eta_hall));
+        if (p.ambipolar) eta = mx(eta, std::fabs(p.eta_ambipolar));
+        ph.has_diff = eta > 0.0 ? 1 : 0;
+        // include/mhd/cfl.hpp:84-88
+        ph.dt_diff = ph.has_diff ? p.cfl * ph.h * ph.h / (2.0 * 2.0 * eta) : 0.0;
+        ph.t_end = t_end;
+        ph.max_steps = max_steps;
+        ph.ni.ohmic = p.ohmic;
+        ph.ni.hall = p.hall;
+        ph.ni.ambipolar = p.ambipolar;
+        ph.ni.eta_ohm = p.eta_ohm;
+        ph.ni.eta_hall = p.eta_hall;
+        ph.ni.eta_ambipolar = p.eta_ambipolar;
+        return ph;
+    }
+
+    static bool nonideal_any(const mhd_hip_params& p)
+    {
+        // include/physics_config/mhd_config.hpp:90-94
+        return (p.ohmic && p.eta_ohm != 0.0) || (p.hall && p.eta_hall != 0.0) ||
+               (p.ambipolar && p.eta_ambipolar != 0.0);
+    }
+
+    template <typename R>
+    void enqueue_step(const Physics& ph, const Tiling& tl, int limiter, bool ni, int parity)
+    {
+        const bool fill = masked && walls;
+        const bool remap = !fill;
+        const Gate gate{ctrl, ph.t_end, ph.max_steps, parity};
+        R* u = static_cast<R*>(U);
+        R* us = static_cast<R*>(Us);
+        StageBuffers b1{u, u, us, masked ? mask_dev : nullptr, ctrl, dt_log, dt_cap};
+        StageBuffers b2{us, u, u, masked ? mask_dev : nullptr, ctrl, dt_log, dt_cap};
+        if (fill) {
+            check(mhdhip::launch_ghost_fill<R>(u, geom, gate, stream), "ghost_fill");
+            check(mhdhip::launch_wall_fill<R>(u, geom, plan_dev, plan_count, ph.gamma, gate, stream),
+                  "wall_fill");
+        }
+        check(mhdhip::launch_stage<R>(1, limiter, ni, remap, b1, geom, ph, tl, parity, stream),
+              "stage1");
+        if (fill) {
+            check(mhdhip::launch_ghost_fill<R>(us, geom, gate, stream), "ghost_fill");
+            check(mhdhip::launch_wall_fill<R>(us, geom, plan_dev, plan_count, ph.gamma, gate,
+                                              stream),
+                  "wall_fill");
+        }
+        check(mhdhip::launch_stage<R>(2, limiter, ni, remap, b2, geom, ph, tl, parity, stream),
+              "stage2");
+    }
+
+    void invalidate_graph()
+    {
+        if (graph_exec) {
+            hipGraphExecDestroy(graph_exec);
+            graph_exec = nullptr;
+        }
+        graph_key.clear();
+    }
+
+    template <typename R>
+    void ensure_graph(const Physics& ph, const Tiling& tl, int limiter, bool ni)
+    {
+        std::vector<unsigned char> key(sizeof(Physics) + sizeof(Tiling) + 2 * sizeof(int) +
+                                       sizeof(void*));
+        unsigned char* k = key.data();
+        std::memcpy(k, &ph, sizeof(Physics));
+        k += sizeof(Physics);
+        std::memcpy(k, &tl, sizeof(Tiling));
+        k += sizeof(Tiling);
+        const int flags[2] = {limiter, ni ? 1 : 0};
+        std::memcpy(k, flags, sizeof(flags));
+        k += sizeof(flags);
+        std::memcpy(k, &dt_log, sizeof(void*));
+        if (graph_exec && key == graph_key) {
+            return;
+        }
+        invalidate_graph();
+        hipGraph_t graph = nullptr;
+        check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "BeginCapture");
+        try {
+            for (int s = 0; s < kSteps; ++s) {
+                enqueue_step<R>(ph, tl, limiter, ni, s & 1);
+            }
+        } catch (...) {
+            hipStreamEndCapture(stream, &graph);
+            if (graph) hipGraphDestroy(graph);
+            throw;
+        }
+        check(hipStreamEndCapture(stream, &graph), "EndCapture");
+        const hipError_t e = hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        check(e, "hipGraphInstantiate");
+        graph_key = key;
+    }
+
+    template <typename R>
+    mhd_hip_result run(const mhd_hip_params& p, double t_end, int max_steps, bool allow_max_steps)
+    {
+        if (t_end < 0.0) {
+            throw InvalidArgument("t_end must be non-negative");
+        }
+        if (p.cfl <= 0.0) {
+            throw InvalidArgument("CFL number must be positive");
+        }
+        if (p.limiter != MHD_HIP_MINMOD && p.limiter != MHD_HIP_MC) {
+            throw InvalidArgument("unknown slope limiter");
+        }
+        if (max_steps < 0) {
+            throw InvalidArgument("max_steps must be non-negative");
+        }
+        hipSetDevice(device);
+        const Physics ph = physics(p, t_end, max_steps);
+        const bool ni = nonideal_any(p);
+        const Tiling tl =
+            mhdhip::make_tiling(grid.nx, grid.ny,
+                                seg_rows > 0 ? seg_rows : mhdhip::default_seg_rows(grid.nx, grid.ny));
+        R* u = static_cast<R*>(U);
+
+        check(hipEventRecord(ev_start, stream), "hipEventRecord");
+        check(mhdhip::launch_ctrl_init<R>(ctrl, stream), "ctrl_init");
+        check(mhdhip::launch_cfl<R>(u, geom, ph, masked ? mask_dev : nullptr, ctrl, stream), "cfl");
+
+        long long launched = 0;
+        const bool any_step = t_end > 0.0 && max_steps > 0;
+        if (any_step) {
+            ensure_graph<R>(ph, tl, p.limiter, ni);
+            // Two replays in flight: queue replay k+1, then inspect the control block after k.
+            hipEvent_t done[2];
+            check(hipEventCreateWithFlags(&done[0], hipEventDisableTiming), "hipEventCreate");
+            check(hipEventCreateWithFlags(&done[1], hipEventDisableTiming), "hipEventCreate");
+            Ctrl* host_slots = nullptr;
+            check(hipHostMalloc(reinterpret_cast<void**>(&host_slots), 2 * sizeof(Ctrl), 0),
+                  "hipHostMalloc");
+            auto launch_one = [&](int slot) {
+                check(hipGraphLaunch(graph_exec, stream), "hipGraphLaunch");
+                launched += kSteps;
+                check(hipMemcpyAsync(&host_slots[slot], ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost,
+                                     stream),
+                      "hipMemcpyAsync(ctrl)");
+                check(hipEventRecord(done[slot], stream), "hipEventRecord");
+            };
+            try {
+                // steps whose count is fixed need no polling beyond the launch count
+                const long long fixed = allow_max_steps
+                                            ? (static_cast<long long>(max_steps) + kSteps - 1) / kSteps
+                                            : -1;
+                int slot = 0;
+                long long issued = 0;
+                launch_one(slot);
+                ++issued;
+                while (true) {
+                    if (fixed >= 0 && issued >= fixed) {
+                        break;
+                    }
+                    launch_one(slot ^ 1);
+                    ++issued;
+                    check(hipEventSynchronize(done[slot]), "hipEventSynchronize");
+                    const Ctrl& c = host_slots[slot];
+                    // parity after the replay that filled this slot is 0 (kSteps is even)
+                    const bool finished =
+                        c.bad_dt != 0 || !(c.t[0] < t_end && c.steps[0] < max_steps);
+                    if (finished && fixed < 0) {
+                        break;
+                    }
+                    slot ^= 1;
+                }
+                check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+            } catch (...) {
+                hipStreamSynchronize(stream);
+                hipHostFree(host_slots);
+                hipEventDestroy(done[0]);
+                hipEventDestroy(done[1]);
+                throw;
+            }
+            hipHostFree(host_slots);
+            hipEventDestroy(done[0]);
+            hipEventDestroy(done[1]);
+        }
+        // final ghost + wall fill: include/mhd/time_integrator.hpp:101-104
+        const Gate open{ctrl, 0.0, 0, -1};
+        check(mhdhip::launch_ghost_fill<R>(u, geom, open, stream), "ghost_fill");
+        if (masked && walls) {
+            check(mhdhip::launch_wall_fill<R>(u, geom, plan_dev, plan_count, p.gamma, open, stream),
+                  "wall_fill");
+        }
+        check(hipEventRecord(ev_stop, stream), "hipEventRecord");
+        check(hipMemcpyAsync(ctrl_host, ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, stream),
+              "hipMemcpyAsync(ctrl)");
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        float ms = 0.0f;
+        check(hipEventElapsedTime(&ms, ev_start, ev_stop), "hipEventElapsedTime");
+        last_seconds = static_cast<double>(ms) * 1e-3;
+
+        const int q = static_cast<int>(launched & 1);
+        mhd_hip_result r{};
+        r.t = ctrl_host->t[q];
+        r.steps = ctrl_host->steps[q];
+        r.c_h = ctrl_host->ch_last[q];
+        dt_count = std::min(r.steps, dt_cap);
+        if (ctrl_host->bad_dt != 0) {
+            throw InvalidArgument("ssp_rk2_step requires positive dt");
+        }
+        if (!allow_max_steps && r.t < t_end && r.steps >= max_steps) {
+            throw std::runtime_error("solve: reached max_steps before t_end");
+        }
+        return r;
+    }
+
+    mhd_hip_result solve(const mhd_hip_params& p)
+    {
+        return precision == MHD_HIP_FP64 ? run<double>(p, p.t_end, p.max_steps, false)
+                                         : run<float>(p, p.t_end, p.max_steps, false);
+    }
+
+    mhd_hip_result advance(const mhd_hip_params& p, int steps)
+    {
+        if (steps < 0) {
+            throw InvalidArgument("steps must be non-negative");
+        }
+        const double inf = std::numeric_limits<double>::infinity();
+        return precision == MHD_HIP_FP64 ? run<double>(p, inf, steps, true)
+                                         : run<float>(p, inf, steps, true);
+    }
+
+    template <typename R>
+    void fill_boundaries_t(double gamma, int what)
+    {
+        const Gate open{ctrl, 0.0, 0, -1};
+        R* u = static_cast<R*>(U);
+        if (what & MHD_HIP_FILL_GHOSTS) {
+            check(mhdhip::launch_ghost_fill<R>(u, geom, open, stream), "ghost_fill");
+        }
+        if ((what & MHD_HIP_FILL_WALLS) && masked && walls) {
+            check(mhdhip::launch_wall_fill<R>(u, geom, plan_dev, plan_count, gamma, open, stream),
+                  "wall_fill");
+        }
+    }
+
+    void fill_boundaries(double gamma, int what)
+    {
+        hipSetDevice(device);
+        if (precision == MHD_HIP_FP64) {
+            fill_boundaries_t<double>(gamma, what);
+        } else {
+            fill_boundaries_t<float>(gamma, what);
+        }
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    }
+
+    void set_dt_log(int cap)
+    {
+        if (cap < 0) {
+            throw InvalidArgument("dt log capacity must be non-negative");
+        }
+        hipSetDevice(device);
+        if (dt_log) {
+            hipFree(dt_log);
+            dt_log = nullptr;
+        }
+        dt_cap = cap;
+        dt_count = 0;
+        if (cap > 0) {
+            check(hipMalloc(reinterpret_cast<void**>(&dt_log), static_cast<size_t>(cap) * sizeof(double)),
+                  "hipMalloc(dt_log)");
+        }
+        invalidate_graph();
+    }
+};
+
+namespace mhdhip {
+
+int default_seg_rows(int nx, int ny)
+{
+    // aim for ~8192 waves; segments of at least 8 rows amortise the 4-row halo
+    const long long strips = (nx + kStripCells - 1) / kStripCells;
+    long long rows = strips * ny / 8192;
+    int r = 8;
+    while (r < rows && r < 128) {
+        r *= 2;
+    }
+    return std::min(r, std::max(1, ny));
+}
+
+Tiling make_tiling(int nx, int ny, int seg_rows)
+{
+    Tiling t{};
+    t.strips = (nx + kStripCells - 1) / kStripCells;
+    t.seg_rows = std::max(1, seg_rows);
+    t.segs = (ny + t.seg_rows - 1) / t.seg_rows;
+    return t;
+}
+
+}  // namespace mhdhip
+
+extern "C" {
+
+int mhd_hip_available(void)
+{
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count < 1) {
+        return 0;
+    }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        return 0;
+    }
+    return device_is_gfx950(dev) ? 1 : 0;
+}
+
+const char* mhd_hip_last_error(void)
+{
+    return g_last_error.c_str();
+}
+
+int mhd_hip_create(const mhd_hip_grid* grid, int precision, mhd_hip_solver** out)
+{
+    return guarded([&] {
+        if (grid == nullptr || out == nullptr) {
+            throw InvalidArgument("null argument");
+        }
+        *out = nullptr;
+        auto s = std::make_unique<mhd_hip_solver>();
+        s->device = -1;
+        s->init(*grid, precision);
+        *out = s.release();
+    });
+}
+
+void mhd_hip_destroy(mhd_hip_solver* s)
+{
+    delete s;
+}
+
+int mhd_hip_device_name(const mhd_hip_solver* s, char* buf, size_t len)
+{
+    return guarded([&] {
+        if (buf == nullptr || len == 0) {
+            throw InvalidArgument("null buffer");
+        }
+        int dev = 0;
+        if (s != nullptr) {
+            dev = s->device;
+        } else {
+            check(hipGetDevice(&dev), "hipGetDevice");
+        }
+        hipDeviceProp_t prop;
+        check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+        std::snprintf(buf, len, "HIP: %s (%s)", prop.name, prop.gcnArchName);
+    });
+}
+
+int mhd_hip_precision(const mhd_hip_solver* s)
+{
+    return s ? s->precision : -1;
+}
+
+int mhd_hip_configure(mhd_hip_solver* s, int bc_x, int bc_y, const uint8_t* mask, size_t mask_len,
+                      int reflective_walls)
+{
+    return guarded([&] {
+        if (!s) throw InvalidArgument("null solver");
+        s->configure(bc_x, bc_y, mask, mask_len, reflective_walls);
+    });
+}
+
+int mhd_hip_upload(mhd_hip_solver* s, const double* const comps[9], size_t n)
+{
+    return guarded([&] {
+        if (!s || !comps) throw InvalidArgument("null argument");
+        for (int c = 0; c < 9; ++c) {
+            if (!comps[c]) throw InvalidArgument("null component pointer");
+        }
+        s->upload(comps, n);
+    });
+}
+
+int mhd_hip_download(mhd_hip_solver* s, double* const comps[9], size_t n)
+{
+    return guarded([&] {
+        if (!s || !comps) throw InvalidArgument("null argument");
+        for (int c = 0; c < 9; ++c) {
+            if (!comps[c]) throw InvalidArgument("null component pointer");
+        }
+        s->download(comps, n);
+    });
+}
+
+int mhd_hip_solve(mhd_hip_solver* s, const mhd_hip_params* params, mhd_hip_result* result)
+{
+    return guarded([&] {
+        if (!s || !params || !result) throw InvalidArgument("null argument");
+        *result = s->solve(*params);
+    });
+}
+
+int mhd_hip_advance_steps(mhd_hip_solver* s, const mhd_hip_params* params, int steps,
+                          mhd_hip_result* result)
+{
+    return guarded([&] {
+        if (!s || !params || !result) throw InvalidArgument("null argument");
+        *result = s->advance(*params, steps);
+    });
+}
+
+int mhd_hip_set_dt_log(mhd_hip_solver* s, int capacity)
+{
+    return guarded([&] {
+        if (!s) throw InvalidArgument("null solver");
+        s->set_dt_log(capacity);
+    });
+}
+
+int mhd_hip_dt_log(mhd_hip_solver* s, double* out, int capacity, int* count)
+{
+    return guarded([&] {
+        if (!s || !count) throw InvalidArgument("null argument");
+        const int n = std::min(capacity, s->dt_count);
+        *count = std::max(0, n);
+        if (n > 0) {
+            if (!out) throw InvalidArgument("null output buffer");
+            hipSetDevice(s->device);
+            check(hipMemcpy(out, s->dt_log, static_cast<size_t>(n) * sizeof(double),
+                            hipMemcpyDeviceToHost),
+                  "hipMemcpy(dt_log)");
+        }
+    });
+}
+
+int mhd_hip_fill_boundaries(mhd_hip_solver* s, double gamma, int what)
+{
+    return guarded([&] {
+        if (!s) throw InvalidArgument("null solver");
+        s->fill_boundaries(gamma, what);
+    });
+}
+
+int mhd_hip_last_solve_seconds(const mhd_hip_solver* s, double* seconds)
+{
+    return guarded([&] {
+        if (!s || !seconds) throw InvalidArgument("null argument");
+        *seconds = s->last_seconds;
+    });
+}
+
+int mhd_hip_init_ic(const char* name, const mhd_hip_grid* grid, double gamma, double b0,
+                    uint64_t seed, double* const comps[9])
+{
+    return guarded([&] {
+        if (!grid || !comps) throw InvalidArgument("null argument");
+        if (grid->nx < 1 || grid->ny < 1 || grid->ng < 0) throw InvalidArgument("bad grid");
+        for (int c = 0; c < 9; ++c) {
+            if (!comps[c]) throw InvalidArgument("null component pointer");
+        }
+        mhdhip::init_ic(name, *grid, gamma, b0, seed, comps);
+    });
+}
+
+int mhd_hip_wall_plan(const mhd_hip_grid* grid, const uint8_t* mask, size_t mask_len,
+                      int64_t* plan, size_t plan_capacity, size_t* count)
+{
+    return guarded([&] {
+        if (!grid || !mask || !count) throw InvalidArgument("null argument");
+        if (mask_len != static_cast<size_t>(grid->nx) * grid->ny) {
+            throw InvalidArgument("FluidMask size does not match Grid2D interior");
+        }
+        std::vector<long long> p;
+        const size_t n = mhdhip::wall_plan(*grid, mask, p);
+        *count = n;
+        if (plan != nullptr) {
+            if (plan_capacity < p.size()) throw InvalidArgument("plan buffer too small");
+            for (size_t k = 0; k < p.size(); ++k) {
+                plan[k] = p[k];
+            }
+        }
+    });
+}
+
+}  // extern "C"
