@@ -136,6 +136,13 @@ int mhd_hip_fill_boundaries(mhd_hip_solver* s, double gamma, int what);
  * stream around the step loop, excluding upload/download). */
 int mhd_hip_last_solve_seconds(const mhd_hip_solver* s, double* seconds);
 
+/* Kernel timing for measurement: while enabled, fixed-step calls (mhd_hip_advance_steps) launch
+ * their steps eagerly (no graph) with a HIP event pair on the solver's stream around each
+ * stage kernel. mhd_hip_kernel_times returns, for the stage-1 and stage-2 kernels, the summed
+ * event time in ms and the number of launches timed since timing was (re)enabled. */
+int mhd_hip_set_kernel_timing(mhd_hip_solver* s, int enable);
+int mhd_hip_kernel_times(const mhd_hip_solver* s, double total_ms[2], long long launches[2]);
+
 /* Host-side initial condition (include/initial_condition/inititial_condition.hpp:28-270):
  * writes the conserved interior of the nine planes for a reference IC name (uniform, sine,
  * blast, orszag_tang, sod, brio_wu, rotor, kelvin_helmholtz / kh, alfven / alfven_wave, with

@@ -59,6 +59,9 @@ SIGNATURES = [
                                  C.POINTER(C.c_int)]),
     ("mhd_hip_fill_boundaries", C.c_int, [C.c_void_p, C.c_double, C.c_int]),
     ("mhd_hip_last_solve_seconds", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    ("mhd_hip_set_kernel_timing", C.c_int, [C.c_void_p, C.c_int]),
+    ("mhd_hip_kernel_times", C.c_int, [C.c_void_p, C.POINTER(C.c_double),
+                                       C.POINTER(C.c_longlong)]),
     ("mhd_hip_init_ic", C.c_int, [C.c_char_p, C.POINTER(Grid), C.c_double, C.c_double,
                                   C.c_uint64, DP9]),
     ("mhd_hip_wall_plan", C.c_int, [C.POINTER(Grid), C.POINTER(C.c_uint8), C.c_size_t,

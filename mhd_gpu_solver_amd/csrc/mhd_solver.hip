@@ -112,6 +112,12 @@ struct mhd_hip_solver {
     int seg_rows = 0;  // 0 = heuristic
     double last_seconds = 0.0;
 
+    // kernel timing mode: fixed-step runs launch eagerly with events around each stage kernel
+    bool kernel_timing = false;
+    double stage_ms[2] = {0.0, 0.0};
+    long long stage_launches[2] = {0, 0};
+    std::vector<hipEvent_t> timing_events;
+
     hipGraphExec_t graph_exec = nullptr;
     std::vector<unsigned char> graph_key;
 
@@ -130,6 +136,8 @@ struct mhd_hip_solver {
         if (dt_log) hipFree(dt_log);
         if (mask_dev) hipFree(mask_dev);
         if (plan_dev) hipFree(plan_dev);
+        for (hipEvent_t e : timing_events) hipEventDestroy(e);
+        timing_events.clear();
         if (ev_start) hipEventDestroy(ev_start);
         if (ev_stop) hipEventDestroy(ev_stop);
         if (stream) hipStreamDestroy(stream);
@@ -362,6 +370,64 @@ struct mhd_hip_solver {
               "stage2");
     }
 
+    // Eager launches of `steps` steps with an event pair around every stage kernel; adds the
+    // kernel times to stage_ms. Returns the number of steps launched (even).
+    template <typename R>
+    long long timed_steps(const Physics& ph, const Tiling& tl, int limiter, bool ni, int steps)
+    {
+        const long long n = steps + (steps & 1);
+        const size_t need = static_cast<size_t>(4 * n);
+        while (timing_events.size() < need) {
+            hipEvent_t e;
+            check(hipEventCreate(&e), "hipEventCreate");
+            timing_events.push_back(e);
+        }
+        const bool fill = masked && walls;
+        const bool remap = !fill;
+        R* u = static_cast<R*>(U);
+        R* us = static_cast<R*>(Us);
+        StageBuffers b1{u, u, us, masked ? mask_dev : nullptr, ctrl, dt_log, dt_cap};
+        StageBuffers b2{us, u, u, masked ? mask_dev : nullptr, ctrl, dt_log, dt_cap};
+        for (long long k = 0; k < n; ++k) {
+            const int parity = static_cast<int>(k & 1);
+            const Gate gate{ctrl, ph.t_end, ph.max_steps, parity};
+            hipEvent_t* e = &timing_events[4 * k];
+            if (fill) {
+                check(mhdhip::launch_ghost_fill<R>(u, geom, gate, stream), "ghost_fill");
+                check(mhdhip::launch_wall_fill<R>(u, geom, plan_dev, plan_count, ph.gamma, gate,
+                                                  stream),
+                      "wall_fill");
+            }
+            check(hipEventRecord(e[0], stream), "hipEventRecord");
+            check(mhdhip::launch_stage<R>(1, limiter, ni, remap, b1, geom, ph, tl, parity, stream),
+                  "stage1");
+            check(hipEventRecord(e[1], stream), "hipEventRecord");
+            if (fill) {
+                check(mhdhip::launch_ghost_fill<R>(us, geom, gate, stream), "ghost_fill");
+                check(mhdhip::launch_wall_fill<R>(us, geom, plan_dev, plan_count, ph.gamma, gate,
+                                                  stream),
+                      "wall_fill");
+            }
+            check(hipEventRecord(e[2], stream), "hipEventRecord");
+            check(mhdhip::launch_stage<R>(2, limiter, ni, remap, b2, geom, ph, tl, parity, stream),
+                  "stage2");
+            check(hipEventRecord(e[3], stream), "hipEventRecord");
+        }
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        for (long long k = 0; k < steps; ++k) {
+            float a = 0.0f, b = 0.0f;
+            check(hipEventElapsedTime(&a, timing_events[4 * k], timing_events[4 * k + 1]),
+                  "hipEventElapsedTime");
+            check(hipEventElapsedTime(&b, timing_events[4 * k + 2], timing_events[4 * k + 3]),
+                  "hipEventElapsedTime");
+            stage_ms[0] += a;
+            stage_ms[1] += b;
+            stage_launches[0] += 1;
+            stage_launches[1] += 1;
+        }
+        return n;
+    }
+
     void invalidate_graph()
     {
         if (graph_exec) {
@@ -436,7 +502,9 @@ struct mhd_hip_solver {
 
         long long launched = 0;
         const bool any_step = t_end > 0.0 && max_steps > 0;
-        if (any_step) {
+        if (any_step && kernel_timing && allow_max_steps) {
+            launched = timed_steps<R>(ph, tl, p.limiter, ni, max_steps);
+        } else if (any_step) {
             ensure_graph<R>(ph, tl, p.limiter, ni);
             // Two replays in flight: queue replay k+1, then inspect the control block after k.
             hipEvent_t done[2];
@@ -745,6 +813,27 @@ int mhd_hip_fill_boundaries(mhd_hip_solver* s, double gamma, int what)
     return guarded([&] {
         if (!s) throw InvalidArgument("null solver");
         s->fill_boundaries(gamma, what);
+    });
+}
+
+int mhd_hip_set_kernel_timing(mhd_hip_solver* s, int enable)
+{
+    return guarded([&] {
+        if (!s) throw InvalidArgument("null solver");
+        s->kernel_timing = enable != 0;
+        s->stage_ms[0] = s->stage_ms[1] = 0.0;
+        s->stage_launches[0] = s->stage_launches[1] = 0;
+    });
+}
+
+int mhd_hip_kernel_times(const mhd_hip_solver* s, double* total_ms, long long* launches)
+{
+    return guarded([&] {
+        if (!s || !total_ms || !launches) throw InvalidArgument("null argument");
+        for (int k = 0; k < 2; ++k) {
+            total_ms[k] = s->stage_ms[k];
+            launches[k] = s->stage_launches[k];
+        }
     });
 }
 

@@ -210,6 +210,16 @@ class DeviceSolver:
         what = (_lib.FILL_GHOSTS if ghosts else 0) | (_lib.FILL_WALLS if walls else 0)
         _lib.check(self._l.mhd_hip_fill_boundaries(self.handle, gamma, what))
 
+    def set_kernel_timing(self, enable: bool):
+        _lib.check(self._l.mhd_hip_set_kernel_timing(self.handle, int(enable)))
+
+    def kernel_times(self):
+        """[(total_ms, launches) for the stage-1 kernel, same for stage 2]"""
+        ms = (C.c_double * 2)()
+        n = (C.c_longlong * 2)()
+        _lib.check(self._l.mhd_hip_kernel_times(self.handle, ms, n))
+        return [(ms[0], n[0]), (ms[1], n[1])]
+
     def last_solve_seconds(self) -> float:
         s = C.c_double()
         _lib.check(self._l.mhd_hip_last_solve_seconds(self.handle, C.byref(s)))
