@@ -78,6 +78,7 @@ def gpu_leg(m, nx, ny, precision, steps, warmup, U0):
     dev.advance_steps(p, steps)
     wall = time.perf_counter() - t0
     k = dev.kernel_times()
+    rows = dev.seg_rows()
     dev.set_kernel_timing(False)
     U = np.zeros_like(U0)
     dev.download(U)
@@ -85,7 +86,7 @@ def gpu_leg(m, nx, ny, precision, steps, warmup, U0):
         raise RuntimeError(f"{precision}: non-finite state after the timed steps")
     cells = nx * ny
     return dict(mcups=cells * steps / wall / 1e6, ms_per_step=wall * 1e3 / steps,
-                graph_mcups=cells * steps / graph_s / 1e6, kernels=k, wall=wall)
+                graph_mcups=cells * steps / graph_s / 1e6, kernels=k, wall=wall, seg_rows=rows)
 
 
 def roofline(leg, nx, ny, elem_bytes):
@@ -107,6 +108,7 @@ def roofline(leg, nx, ny, elem_bytes):
         "traffic": None,
         "bytes_per_launch": bytes_launch,
         "avg_launch_ms": round(avg[dom], 4),
+        "seg_rows": leg["seg_rows"],
         "stage_avg_ms": [round(a, 4) for a in avg],
         "step_achieved_gbs": round(step_gbs, 1),
         "step_frac": round(step_gbs / HBM_PEAK_GBS, 4),
@@ -147,6 +149,24 @@ def cpu_leg(m, nx, ny, U0, budget_s=12.0):
     }
 
 
+def copy_probe(nbytes, repeats=3):
+    """Plain device-to-device copies of known size (hipMemcpy's copy kernel): the scale for the
+    FETCH_SIZE / WRITE_SIZE counters of a profiling run."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed in copy probe")
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    for _ in range(repeats):
+        if hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3):  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed in copy probe")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy probe: {repeats} x {nbytes} B device-to-device")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +175,8 @@ def main():
     ap.add_argument("--nx", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="also run plain device copies of BYTES (counter scale for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -165,6 +187,8 @@ def main():
 
     if not m.hip_available():
         raise SystemExit("no gfx950 device visible to the hip backend")
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
     nx = ny = args.nx
     log(f"building initial state {nx}x{ny}")
     U0 = initial_state(m, nx, ny)

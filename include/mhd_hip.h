@@ -136,6 +136,13 @@ int mhd_hip_fill_boundaries(mhd_hip_solver* s, double gamma, int what);
  * stream around the step loop, excluding upload/download). */
 int mhd_hip_last_solve_seconds(const mhd_hip_solver* s, double* seconds);
 
+/* Rows of the grid marched by one wavefront of the stage kernel (the tiling's only free
+ * parameter). 0 (default) selects the autotuned value: on the first solve of a shape the
+ * stage kernel is timed for candidate lengths and the fastest is kept for the process.
+ * Results do not depend on this value. mhd_hip_get_seg_rows reports the value last used. */
+int mhd_hip_set_seg_rows(mhd_hip_solver* s, int rows);
+int mhd_hip_get_seg_rows(mhd_hip_solver* s, int* rows);
+
 /* Kernel timing for measurement: while enabled, fixed-step calls (mhd_hip_advance_steps) launch
  * their steps eagerly (no graph) with a HIP event pair on the solver's stream around each
  * stage kernel. mhd_hip_kernel_times returns, for the stage-1 and stage-2 kernels, the summed

@@ -91,6 +91,20 @@ __device__ __forceinline__ bool ctrl_pending(const Ctrl* ctrl, const Physics& ph
     return ctrl->t[p] < ph.t_end && ctrl->steps[p] < ph.max_steps;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (physical id p runs on XCD p % 8).
+// Returns a logical id such that each XCD receives a contiguous range of logical ids, in the
+// order its workgroups are dispatched.
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ int xcd_logical_block(int p, int nb)
+{
+    const int q = nb / kXcds;
+    const int r = nb % kXcds;
+    const int xcd = p % kXcds;
+    const int local = p / kXcds;
+    return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
+}
+
 template <typename R>
 __device__ __forceinline__ R wave_max(R v)
 {
@@ -310,7 +324,7 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageBuffers b, Geom g, Physic
                                                   int parity)
 {
     const StepState st = step_state(b.ctrl, ph, parity);
-    const bool lead = blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     if (STAGE == 1 && lead) {
         b.ctrl->chmax[parity ^ 1] = 0ull;
         if (!st.active && ctrl_pending(b.ctrl, ph, parity)) {
@@ -329,10 +343,14 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageBuffers b, Geom g, Physic
     if (!st.active) {
         return;
     }
-    // block (strip, segment group): its four waves march four consecutive row segments
+    // block -> (strip, segment group); its four waves march four consecutive row segments.
+    // Consecutive logical blocks (neighbouring strips) are placed on the same XCD so that the
+    // halo columns and the cache lines shared across a strip edge meet in one L2.
     const int lane = threadIdx.x & (kWave - 1);
-    const int strip = blockIdx.x;
-    const int seg = blockIdx.y * kWavesPerBlock + (threadIdx.x >> 6);
+    const int wave = threadIdx.x >> 6;
+    const int logical = xcd_logical_block(blockIdx.x, gridDim.x);
+    const int strip = logical % tl.strips;
+    const int seg = (logical / tl.strips) * kWavesPerBlock + wave;
     if (strip >= tl.strips || seg >= tl.segs) {
         return;
     }
@@ -379,6 +397,16 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageBuffers b, Geom g, Physic
     R Jp[3] = {R(0), R(0), R(0)};
     R local_speed = R(0);
 
+    // wave-private ring of the last three raw (conserved) rows: the RK update of row c-1 reads
+    // its stage input from here instead of global memory
+    __shared__ R ring[kWavesPerBlock][3][kNC][kWave];
+    auto ring_put = [&](int q, const R (&u)[kNC]) {
+#pragma unroll
+        for (int k = 0; k < kNC; ++k) {
+            ring[wave][q % 3][k][lane] = u[k];
+        }
+    };
+
     load_row(r0 - 2, Uraw);
     to_prim(Uraw, gm1, Wa);
     load_row(r0 - 1, Uraw);
@@ -386,6 +414,7 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageBuffers b, Geom g, Physic
     load_row(r0, Uraw);
 
     for (int c = r0 - 1; c <= r1; ++c) {
+        ring_put(c + 1, Uraw);
         to_prim(Uraw, gm1, Wn);  // row c + 1
         if (c + 2 <= r1 + 1) {
             load_row(c + 2, Uraw);
@@ -476,18 +505,20 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageBuffers b, Geom g, Physic
                         }
                     }
                     const long long cell = static_cast<long long>(rr) * g.nxt + gi;
+                    const int slot = rr % 3;
                     if (STAGE == 1) {
                         // include/mhd/field_ops.hpp:47-53
 #pragma unroll
                         for (int k = 0; k < kNC; ++k) {
-                            dst[k * P + cell] = src[k * P + cell] + dt * L[k];
+                            dst[k * P + cell] = ring[wave][slot][k][lane] + dt * L[k];
                         }
                     } else {
                         // include/mhd/field_ops.hpp:55-63
                         R un[kNC];
 #pragma unroll
                         for (int k = 0; k < kNC; ++k) {
-                            un[k] = half * (u0[k * P + cell] + src[k * P + cell]) + half_dt * L[k];
+                            un[k] = half * (u0[k * P + cell] + ring[wave][slot][k][lane]) +
+                                    half_dt * L[k];
                         }
 #pragma unroll
                         for (int k = 0; k < kNC; ++k) {

@@ -11,9 +11,9 @@ namespace detail {
 
 inline dim3 stage_grid(const Tiling& tl)
 {
-    // blockIdx.x -> strip, blockIdx.y * waves + wave -> row segment
-    return dim3(static_cast<unsigned>(tl.strips),
-                static_cast<unsigned>((tl.segs + kWavesPerBlock - 1) / kWavesPerBlock), 1);
+    // one block per (strip, group of four row segments); flat grid, remapped in the kernel
+    const long long groups = (tl.segs + kWavesPerBlock - 1) / kWavesPerBlock;
+    return dim3(static_cast<unsigned>(tl.strips * groups), 1, 1);
 }
 
 inline unsigned flat_blocks(long long work, int block, int cap)

@@ -15,9 +15,13 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <memory>
+#include <mutex>
+#include <tuple>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -25,6 +29,18 @@
 namespace {
 
 thread_local std::string g_last_error;
+
+// Process-wide autotuning results: best rows-per-wave for a (shape, precision, kernel variant).
+struct TuneKey {
+    int nx, ny, precision, limiter, ni, remap;
+    bool operator<(const TuneKey& o) const
+    {
+        return std::tie(nx, ny, precision, limiter, ni, remap) <
+               std::tie(o.nx, o.ny, o.precision, o.limiter, o.ni, o.remap);
+    }
+};
+std::mutex g_tune_mutex;
+std::map<TuneKey, int> g_tuned;
 
 struct InvalidArgument : std::invalid_argument {
     using std::invalid_argument::invalid_argument;
@@ -109,7 +125,8 @@ struct mhd_hip_solver {
     long long* plan_dev = nullptr;
     int plan_count = 0;
 
-    int seg_rows = 0;  // 0 = heuristic
+    int seg_rows = 0;   // rows marched per wave; 0 = autotuned per shape
+    int last_rows = 0;  // value used by the last solve
     double last_seconds = 0.0;
 
     // kernel timing mode: fixed-step runs launch eagerly with events around each stage kernel
@@ -428,6 +445,76 @@ struct mhd_hip_solver {
         return n;
     }
 
+    // Autotuner: times the stage-1 kernel (which leaves U untouched) for candidate row-segment
+    // lengths on the current state and keeps the fastest per shape and kernel variant. Runs
+    // after the first CFL reduction of a solve, so the kernels see a live step.
+    template <typename R>
+    int tuned_rows(const Physics& ph, int limiter, bool ni)
+    {
+        const bool remap = !(masked && walls);
+        const TuneKey key{grid.nx, grid.ny, precision, limiter, ni ? 1 : 0, remap ? 1 : 0};
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mutex);
+            auto it = g_tuned.find(key);
+            if (it != g_tuned.end()) {
+                return it->second;
+            }
+        }
+        if (const char* env = std::getenv("MHD_HIP_SEG_ROWS")) {
+            const int v = std::atoi(env);
+            if (v > 0) {
+                return v;
+            }
+        }
+        if (static_cast<long long>(grid.nx) * grid.ny < 4096) {
+            return mhdhip::default_seg_rows(grid.nx, grid.ny);
+        }
+        R* u = static_cast<R*>(U);
+        R* us = static_cast<R*>(Us);
+        StageBuffers b1{u, u, us, masked ? mask_dev : nullptr, ctrl, nullptr, 0};
+        hipEvent_t e0, e1;
+        check(hipEventCreate(&e0), "hipEventCreate");
+        check(hipEventCreate(&e1), "hipEventCreate");
+        int best = mhdhip::default_seg_rows(grid.nx, grid.ny);
+        float best_ms = 1e30f;
+        try {
+            for (int cand : {8, 16, 24, 32, 48, 64, 96, 128}) {
+                if (cand > grid.ny && cand != 8) {
+                    continue;
+                }
+                const Tiling tl = mhdhip::make_tiling(grid.nx, grid.ny, cand);
+                // one untimed launch, then the best of three
+                check(mhdhip::launch_stage<R>(1, limiter, ni, remap, b1, geom, ph, tl, 0, stream),
+                      "tune");
+                float cand_ms = 1e30f;
+                for (int rep = 0; rep < 3; ++rep) {
+                    check(hipEventRecord(e0, stream), "hipEventRecord");
+                    check(mhdhip::launch_stage<R>(1, limiter, ni, remap, b1, geom, ph, tl, 0,
+                                                  stream),
+                          "tune");
+                    check(hipEventRecord(e1, stream), "hipEventRecord");
+                    check(hipEventSynchronize(e1), "hipEventSynchronize");
+                    float ms = 0.0f;
+                    check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+                    cand_ms = std::min(cand_ms, ms);
+                }
+                if (cand_ms < best_ms) {
+                    best_ms = cand_ms;
+                    best = cand;
+                }
+            }
+        } catch (...) {
+            hipEventDestroy(e0);
+            hipEventDestroy(e1);
+            throw;
+        }
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        std::lock_guard<std::mutex> lk(g_tune_mutex);
+        g_tuned[key] = best;
+        return best;
+    }
+
     void invalidate_graph()
     {
         if (graph_exec) {
@@ -491,14 +578,14 @@ struct mhd_hip_solver {
         hipSetDevice(device);
         const Physics ph = physics(p, t_end, max_steps);
         const bool ni = nonideal_any(p);
-        const Tiling tl =
-            mhdhip::make_tiling(grid.nx, grid.ny,
-                                seg_rows > 0 ? seg_rows : mhdhip::default_seg_rows(grid.nx, grid.ny));
         R* u = static_cast<R*>(U);
 
         check(hipEventRecord(ev_start, stream), "hipEventRecord");
         check(mhdhip::launch_ctrl_init<R>(ctrl, stream), "ctrl_init");
         check(mhdhip::launch_cfl<R>(u, geom, ph, masked ? mask_dev : nullptr, ctrl, stream), "cfl");
+        const int rows = seg_rows > 0 ? seg_rows : tuned_rows<R>(ph, p.limiter, ni);
+        const Tiling tl = mhdhip::make_tiling(grid.nx, grid.ny, rows);
+        last_rows = rows;
 
         long long launched = 0;
         const bool any_step = t_end > 0.0 && max_steps > 0;
@@ -813,6 +900,24 @@ int mhd_hip_fill_boundaries(mhd_hip_solver* s, double gamma, int what)
     return guarded([&] {
         if (!s) throw InvalidArgument("null solver");
         s->fill_boundaries(gamma, what);
+    });
+}
+
+int mhd_hip_set_seg_rows(mhd_hip_solver* s, int rows)
+{
+    return guarded([&] {
+        if (!s) throw InvalidArgument("null solver");
+        if (rows < 0) throw InvalidArgument("rows per wave segment must be >= 0");
+        s->seg_rows = rows;
+        s->invalidate_graph();
+    });
+}
+
+int mhd_hip_get_seg_rows(mhd_hip_solver* s, int* rows)
+{
+    return guarded([&] {
+        if (!s || !rows) throw InvalidArgument("null argument");
+        *rows = s->last_rows;
     });
 }
 

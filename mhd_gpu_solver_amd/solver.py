@@ -210,6 +210,15 @@ class DeviceSolver:
         what = (_lib.FILL_GHOSTS if ghosts else 0) | (_lib.FILL_WALLS if walls else 0)
         _lib.check(self._l.mhd_hip_fill_boundaries(self.handle, gamma, what))
 
+    def set_seg_rows(self, rows: int):
+        """Rows marched per wavefront (0 = autotuned); results do not depend on it."""
+        _lib.check(self._l.mhd_hip_set_seg_rows(self.handle, int(rows)))
+
+    def seg_rows(self) -> int:
+        v = C.c_int()
+        _lib.check(self._l.mhd_hip_get_seg_rows(self.handle, C.byref(v)))
+        return v.value
+
     def set_kernel_timing(self, enable: bool):
         _lib.check(self._l.mhd_hip_set_kernel_timing(self.handle, int(enable)))
 
