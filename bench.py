@@ -89,6 +89,20 @@ def gpu_leg(m, nx, ny, precision, steps, warmup, U0):
                 graph_mcups=cells * steps / graph_s / 1e6, kernels=k, wall=wall, seg_rows=rows)
 
 
+def measured_traffic(kernel_prefix, nx, ny, elem_bytes):
+    """HBM bytes per launch of a stage kernel from the committed FETCH_SIZE/WRITE_SIZE summary
+    (profiles/traffic_k_stage_orszag_tang_<n>.json), or None when none exists for this size."""
+    path = ROOT / "profiles" / f"traffic_k_stage_orszag_tang_{nx}.json"
+    if nx != ny or not path.exists():
+        return None
+    data = json.loads(path.read_text())
+    want = "double" if elem_bytes == 8 else "float"
+    for name, v in data.get("kernels", {}).items():
+        if name.startswith(kernel_prefix) and want in name:
+            return v["traffic_bytes"]
+    return None
+
+
 def roofline(leg, nx, ny, elem_bytes):
     cells = nx * ny
     # algorithmic bytes per launch: stage 1 reads U, writes U*; stage 2 reads U*, U, writes U
@@ -105,7 +119,7 @@ def roofline(leg, nx, ny, elem_bytes):
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": measured_traffic(f"k_stage<{dom + 1}", nx, ny, elem_bytes),
         "bytes_per_launch": bytes_launch,
         "avg_launch_ms": round(avg[dom], 4),
         "seg_rows": leg["seg_rows"],

@@ -1,0 +1,102 @@
+"""Summaries of rocprofv3 CSV output for the hip backend's kernels.
+
+    python -m mhd_gpu_solver_amd.profiling trace  <kernel_trace.csv>
+    python -m mhd_gpu_solver_amd.profiling traffic <fetch counter_collection.csv> \
+        <write counter_collection.csv> <copy bytes> <nx> <ny>
+
+``trace`` prints per-kernel launch statistics. Launches shorter than ``NOOP_US`` are the
+early-exit steps at the end of a fixed-size batch and are counted apart from the active ones.
+``traffic`` converts FETCH_SIZE / WRITE_SIZE (KiB per dispatch) into bytes per launch, scaled
+by what the same counter reads for a plain device copy of known size in the same run.
+"""
+from __future__ import annotations
+
+import collections
+import csv
+import json
+import sys
+
+NOOP_US = 50.0
+
+
+def _short(name: str) -> str:
+    return name.split("(")[0].replace("void ", "").replace("mhdhip::", "")
+
+
+def kernel_table(trace_csv: str) -> str:
+    rows = list(csv.DictReader(open(trace_csv)))
+    durs = collections.defaultdict(list)
+    for r in rows:
+        d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+        durs[_short(r["Kernel_Name"])].append(d)
+    out = ["| kernel | launches | active | mean active (us) | min (us) | max (us) | no-op mean (us) |",
+           "|---|---|---|---|---|---|---|"]
+    for k, v in sorted(durs.items(), key=lambda kv: -sum(kv[1])):
+        act = [x for x in v if x > NOOP_US]
+        noop = [x for x in v if x <= NOOP_US]
+        if act:
+            out.append(f"| {k} | {len(v)} | {len(act)} | {sum(act) / len(act):.1f} | "
+                       f"{min(act):.1f} | {max(act):.1f} | "
+                       f"{(sum(noop) / len(noop)) if noop else 0:.2f} |")
+        else:
+            out.append(f"| {k} | {len(v)} | 0 | - | - | - | {sum(noop) / len(noop):.2f} |")
+    return "\n".join(out)
+
+
+def _counter(path: str, counter: str):
+    per = collections.defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        if r["Counter_Name"] != counter:
+            continue
+        d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+        per[_short(r["Kernel_Name"])].append((float(r["Counter_Value"]), d))
+    return per
+
+
+def traffic(fetch_csv: str, write_csv: str, copy_bytes: int, nx: int, ny: int) -> dict:
+    fetch = _counter(fetch_csv, "FETCH_SIZE")
+    write = _counter(write_csv, "WRITE_SIZE")
+    kib = copy_bytes / 1024.0
+    copy_f = [v for v, d in fetch["__amd_rocclr_copyBuffer"] if v > 0.1 * kib]
+    copy_w = [v for v, d in write["__amd_rocclr_copyBuffer"] if v > 0.1 * kib]
+    f_scale = kib / (sum(copy_f) / len(copy_f))
+    w_scale = kib / (sum(copy_w) / len(copy_w))
+    cells = nx * ny
+    res = {"copy_bytes": copy_bytes, "fetch_scale": round(f_scale, 4),
+           "write_scale": round(w_scale, 4), "kernels": {}}
+    for k in sorted(set(fetch) | set(write)):
+        if "k_stage" not in k:
+            continue
+        f = [v for v, d in fetch.get(k, []) if d > NOOP_US]
+        w = [v for v, d in write.get(k, []) if d > NOOP_US]
+        if not f or not w:
+            continue
+        fb = sum(f) / len(f) * f_scale * 1024
+        wb = sum(w) / len(w) * w_scale * 1024
+        elem = 8 if "double" in k else 4
+        stage = 1 if k.startswith("k_stage<1") else 2
+        alg_r = (9 if stage == 1 else 18) * elem * cells
+        alg_w = 9 * elem * cells
+        res["kernels"][k] = {
+            "fetch_bytes": round(fb), "write_bytes": round(wb), "traffic_bytes": round(fb + wb),
+            "algorithmic_read": alg_r, "algorithmic_write": alg_w,
+            "fetch_over_algorithmic": round(fb / alg_r, 3),
+            "write_over_algorithmic": round(wb / alg_w, 3),
+        }
+    return res
+
+
+def main(argv):
+    if len(argv) >= 2 and argv[0] == "trace":
+        print(kernel_table(argv[1]))
+    elif len(argv) >= 6 and argv[0] == "traffic":
+        print(json.dumps(traffic(argv[1], argv[2], int(argv[3]), int(argv[4]), int(argv[5])),
+                         indent=1))
+    else:
+        print(__doc__)
+        return 2
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))
