@@ -32,7 +32,8 @@ enum mhd_hip_status {
     MHD_HIP_OK = 0,
     MHD_HIP_EINVAL = 1,
     MHD_HIP_ERUNTIME = 2,
-    MHD_HIP_ENODEV = 3
+    MHD_HIP_ENODEV = 3,
+    MHD_HIP_EFULL = 4 /* batch capacity (HBM budget or instance slots) exhausted; retry later */
 };
 
 /* mhd::Precision (device_solver.hpp:28) */
@@ -171,6 +172,71 @@ int mhd_hip_init_ic(const char* name, const mhd_hip_grid* grid, double gamma, do
  * through *count. Runs on the CPU; needs no device. */
 int mhd_hip_wall_plan(const mhd_hip_grid* grid, const uint8_t* mask, size_t mask_len,
                       int64_t* plan, size_t plan_capacity, size_t* count);
+
+/* ---- Batched ensemble engine ------------------------------------------------------------
+ * No counterpart in the reference, which advances one grid per DeviceSolver
+ * (device_solver.hpp:57-75) and runs concurrent solves only on separate host threads
+ * (docs/performance.md:80-81). Many independent grids of different sizes stay resident in an
+ * HBM arena of a caller-given budget and advance together in one persistent kernel per RK
+ * stage, each with its own dt (device-side CFL per instance), gamma, CFL number, end time and
+ * periodic/outflow boundaries; limiter and non-ideal terms are shared by the batch. Each
+ * instance follows the ::solve() contract (time_integrator.hpp:62-106) exactly as a single-run
+ * solver would: its result does not depend on the other instances. */
+
+typedef struct mhd_hip_batch mhd_hip_batch;
+
+/* One solve: grid, boundaries and SolveParams fields that may differ per instance. */
+typedef struct mhd_hip_instance {
+    int nx, ny; /* interior cells; ng is 2 */
+    double x_min, x_max, y_min, y_max;
+    int bc_x, bc_y; /* MHD_HIP_PERIODIC or MHD_HIP_OUTFLOW */
+    double t_end;   /* relative end time; +inf with max_steps for a fixed step count */
+    double cfl, gamma, glm_alpha;
+    int max_steps;
+} mhd_hip_instance;
+
+typedef struct mhd_hip_batch_stats {
+    int instances; /* admitted (running or finished, not yet removed) */
+    int active;    /* still stepping */
+    int tiles;     /* wave work items in the current tile list */
+    int seg_rows;  /* rows per wave in use (autotuned unless set) */
+    int blocks;    /* workgroups of the persistent stage kernel */
+    size_t bytes_used, bytes_budget;
+    double last_seconds; /* device time of the last mhd_hip_batch_advance */
+} mhd_hip_batch_stats;
+
+/* HBM bytes one instance of an nx x ny grid occupies in the arena (U and U*). */
+size_t mhd_hip_batch_instance_bytes(int precision, int nx, int ny);
+
+/* physics: limiter and non-ideal fields are used (shared by all instances); others ignored. */
+int mhd_hip_batch_create(int precision, const mhd_hip_params* physics, size_t hbm_budget_bytes,
+                         int max_instances, mhd_hip_batch** out);
+void mhd_hip_batch_destroy(mhd_hip_batch* b);
+
+/* Admission: copies the full fp64 state (nine planes of (ny+4)*(nx+4), ghosts included) into
+ * the arena and runs the instance's first CFL reduction. MHD_HIP_EFULL when the HBM budget or
+ * the instance slots are exhausted. Instances join at the next replay. */
+int mhd_hip_batch_add(mhd_hip_batch* b, const mhd_hip_instance* inst, const double* const comps[9],
+                      int* id);
+
+/* Releases an instance's slot and memory (finished or not). */
+int mhd_hip_batch_remove(mhd_hip_batch* b, int id);
+
+/* Replays the step graph until no instance is active, or at most max_replays times
+ * (max_replays < 0: no limit). *active receives the number of instances still stepping. */
+int mhd_hip_batch_advance(mhd_hip_batch* b, int max_replays, int* active);
+
+/* Result so far (t, steps, c_h as SolveResult) and whether the instance has finished; the
+ * ::solve() errors are reported here (EINVAL for dt <= 0, ERUNTIME for max_steps before t_end). */
+int mhd_hip_batch_result(mhd_hip_batch* b, int id, mhd_hip_result* result, int* finished);
+
+/* Ghost fill (the solve's final fill) and copy of the instance's state to nine fp64 planes. */
+int mhd_hip_batch_download(mhd_hip_batch* b, int id, double* const comps[9], size_t n);
+
+/* Rows per wave for the batch kernel; 0 (default) autotunes on the first advance. */
+int mhd_hip_batch_set_seg_rows(mhd_hip_batch* b, int rows);
+
+int mhd_hip_batch_info(const mhd_hip_batch* b, mhd_hip_batch_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -12,7 +12,7 @@ import threading
 PKG = pathlib.Path(__file__).resolve().parent
 LIB_PATH = PKG / "libmhd_hip.so"
 
-OK, EINVAL, ERUNTIME, ENODEV = 0, 1, 2, 3
+OK, EINVAL, ERUNTIME, ENODEV, EFULL = 0, 1, 2, 3, 4
 FP64, FP32 = 0, 1
 FILL_GHOSTS, FILL_WALLS = 1, 2
 
@@ -32,6 +32,19 @@ class Params(C.Structure):
 
 class Result(C.Structure):
     _fields_ = [("t", C.c_double), ("steps", C.c_int), ("c_h", C.c_double)]
+
+
+class Instance(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("x_min", C.c_double), ("x_max", C.c_double),
+                ("y_min", C.c_double), ("y_max", C.c_double), ("bc_x", C.c_int),
+                ("bc_y", C.c_int), ("t_end", C.c_double), ("cfl", C.c_double),
+                ("gamma", C.c_double), ("glm_alpha", C.c_double), ("max_steps", C.c_int)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("instances", C.c_int), ("active", C.c_int), ("tiles", C.c_int),
+                ("seg_rows", C.c_int), ("blocks", C.c_int), ("bytes_used", C.c_size_t),
+                ("bytes_budget", C.c_size_t), ("last_seconds", C.c_double)]
 
 
 DP9 = C.POINTER(C.c_double) * 9
@@ -68,7 +81,23 @@ SIGNATURES = [
                                   C.c_uint64, DP9]),
     ("mhd_hip_wall_plan", C.c_int, [C.POINTER(Grid), C.POINTER(C.c_uint8), C.c_size_t,
                                     C.POINTER(C.c_int64), C.c_size_t, C.POINTER(C.c_size_t)]),
+    ("mhd_hip_batch_instance_bytes", C.c_size_t, [C.c_int, C.c_int, C.c_int]),
+    ("mhd_hip_batch_create", C.c_int, [C.c_int, C.POINTER(Params), C.c_size_t, C.c_int,
+                                       C.POINTER(C.c_void_p)]),
+    ("mhd_hip_batch_destroy", None, [C.c_void_p]),
+    ("mhd_hip_batch_add", C.c_int, [C.c_void_p, C.POINTER(Instance), DP9, C.POINTER(C.c_int)]),
+    ("mhd_hip_batch_remove", C.c_int, [C.c_void_p, C.c_int]),
+    ("mhd_hip_batch_advance", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
+    ("mhd_hip_batch_result", C.c_int, [C.c_void_p, C.c_int, C.POINTER(Result),
+                                       C.POINTER(C.c_int)]),
+    ("mhd_hip_batch_download", C.c_int, [C.c_void_p, C.c_int, DP9, C.c_size_t]),
+    ("mhd_hip_batch_set_seg_rows", C.c_int, [C.c_void_p, C.c_int]),
+    ("mhd_hip_batch_info", C.c_int, [C.c_void_p, C.POINTER(BatchStats)]),
 ]
+
+
+class CapacityFull(RuntimeError):
+    """MHD_HIP_EFULL: the batch's HBM budget or instance slots are exhausted; retry later."""
 
 
 def lib() -> C.CDLL:
@@ -100,6 +129,8 @@ def check(rc: int, what: str = "") -> None:
     msg = last_error() or what
     if rc == EINVAL:
         raise ValueError(msg)
+    if rc == EFULL:
+        raise CapacityFull(msg)
     raise RuntimeError(msg)
 
 

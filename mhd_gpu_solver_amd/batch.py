@@ -1,0 +1,102 @@
+"""Batched ensemble engine (include/mhd_hip.h, mhd_hip_batch_*).
+
+Many independent solves of different grid sizes stay resident on the GPU and advance together,
+one persistent kernel per RK stage, each with its own dt. Each instance follows the ::solve()
+contract of the reference (include/mhd/time_integrator.hpp:62-106) and its result does not
+depend on which other instances share the batch.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _lib
+from .solver import NonIdealConfig, SlopeLimiter, SolveResult
+
+CapacityFull = _lib.CapacityFull
+
+
+def instance_bytes(precision: str, nx: int, ny: int) -> int:
+    """HBM bytes one nx x ny instance occupies (U and U*)."""
+    return int(_lib.lib().mhd_hip_batch_instance_bytes(_prec(precision), int(nx), int(ny)))
+
+
+def _prec(precision):
+    if precision not in ("fp64", "fp32"):
+        raise ValueError("precision must be 'fp64' or 'fp32'")
+    return _lib.FP64 if precision == "fp64" else _lib.FP32
+
+
+class BatchEngine:
+    def __init__(self, precision="fp32", limiter=SlopeLimiter.MC, nonideal=None,
+                 hbm_budget=8 << 30, max_instances=1024):
+        self._l = _lib.lib()
+        p = _lib.Params()
+        p.limiter = int(limiter)
+        ni = nonideal or NonIdealConfig()
+        p.ohmic, p.hall, p.ambipolar = int(ni.ohmic), int(ni.hall), int(ni.ambipolar)
+        p.eta_ohm, p.eta_hall, p.eta_ambipolar = ni.eta_ohm, ni.eta_hall, ni.eta_ambipolar
+        h = C.c_void_p()
+        _lib.check(self._l.mhd_hip_batch_create(_prec(precision), C.byref(p), int(hbm_budget),
+                                                int(max_instances), C.byref(h)))
+        self.handle = h
+        self.precision = precision
+        self._shape = {}
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            self._l.mhd_hip_batch_destroy(h)
+            self.handle = None
+
+    def add(self, state, t_end, cfl=0.4, gamma=5.0 / 3.0, glm_alpha=0.1, bc=(0, 0),
+            max_steps=1_000_000, x_min=0.0, x_max=1.0, y_min=0.0, y_max=1.0) -> int:
+        """Admit one solve; `state` is (9, ny+4, nx+4) float64 (ghosts included).
+        Raises CapacityFull when the HBM budget or the slots are exhausted."""
+        U = np.ascontiguousarray(state, dtype=np.float64)
+        ny, nx = U.shape[1] - 4, U.shape[2] - 4
+        d = _lib.Instance(nx, ny, x_min, x_max, y_min, y_max, int(bc[0]), int(bc[1]),
+                          float(t_end), float(cfl), float(gamma), float(glm_alpha),
+                          int(max_steps))
+        ptrs, _ = _lib.planes(U)
+        i = C.c_int()
+        _lib.check(self._l.mhd_hip_batch_add(self.handle, C.byref(d), ptrs, C.byref(i)))
+        self._shape[i.value] = (ny + 4, nx + 4)
+        return i.value
+
+    def add_steps(self, state, steps, **kw) -> int:
+        """Admit a fixed-step-count solve (t_end = +inf, max_steps = steps)."""
+        return self.add(state, t_end=math.inf, max_steps=steps, **kw)
+
+    def remove(self, iid: int):
+        _lib.check(self._l.mhd_hip_batch_remove(self.handle, int(iid)))
+        self._shape.pop(iid, None)
+
+    def advance(self, max_replays: int = -1) -> int:
+        """Step until no instance is active (or max_replays graph replays); returns active."""
+        a = C.c_int()
+        _lib.check(self._l.mhd_hip_batch_advance(self.handle, int(max_replays), C.byref(a)))
+        return a.value
+
+    def result(self, iid: int):
+        r = _lib.Result()
+        f = C.c_int()
+        _lib.check(self._l.mhd_hip_batch_result(self.handle, int(iid), C.byref(r), C.byref(f)))
+        return SolveResult(r.t, r.steps, r.c_h), bool(f.value)
+
+    def download(self, iid: int):
+        shape = self._shape[iid]
+        U = np.zeros((9,) + shape)
+        ptrs, n = _lib.planes(U, writable=True)
+        _lib.check(self._l.mhd_hip_batch_download(self.handle, int(iid), ptrs, n))
+        return U
+
+    def set_seg_rows(self, rows: int):
+        _lib.check(self._l.mhd_hip_batch_set_seg_rows(self.handle, int(rows)))
+
+    def stats(self) -> dict:
+        st = _lib.BatchStats()
+        _lib.check(self._l.mhd_hip_batch_info(self.handle, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in _lib.BatchStats._fields_}

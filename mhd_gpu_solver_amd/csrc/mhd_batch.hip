@@ -1,0 +1,780 @@
+// Batched ensemble engine and its C ABI (include/mhd_hip.h, mhd_hip_batch_*).
+//
+// Many independent solves of different grid sizes share one device: each instance keeps its
+// state (U and U*) resident in an HBM arena sized by the caller's budget, its own control block
+// (t, steps, dt, c_h on the device) and its own physics (gamma, CFL number, end time, boundary
+// types). One persistent kernel per RK stage advances every active instance; a hipGraph of
+// kSteps steps is replayed and the host reads all control blocks once per replay. Instances
+// join (admission, first CFL reduction) and leave between replays by copies on the engine's
+// stream, so the captured graph stays valid across changes of the instance set.
+#include "mhd_abi.hpp"
+#include "mhd_hip.h"
+#include "mhd_launch.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+using mhdhip::BatchInst;
+using mhdhip::BatchTables;
+using mhdhip::BatchTile;
+using mhdhip::check;
+using mhdhip::Ctrl;
+using mhdhip::Full;
+using mhdhip::Gate;
+using mhdhip::Geom;
+using mhdhip::guarded;
+using mhdhip::InvalidArgument;
+using mhdhip::NoDevice;
+using mhdhip::Physics;
+
+namespace {
+
+constexpr size_t kAlign = 256;
+
+size_t round_up(size_t v, size_t a)
+{
+    return (v + a - 1) / a * a;
+}
+
+// First-fit allocator over one device arena (offsets only; the arena is one hipMalloc).
+class Arena {
+  public:
+    void reset(size_t cap)
+    {
+        cap_ = cap;
+        free_.clear();
+        if (cap > 0) {
+            free_[0] = cap;
+        }
+        used_ = 0;
+    }
+    bool alloc(size_t bytes, size_t& off)
+    {
+        bytes = round_up(bytes, kAlign);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                off = it->first;
+                const size_t rest = it->second - bytes;
+                free_.erase(it);
+                if (rest > 0) {
+                    free_[off + bytes] = rest;
+                }
+                used_ += bytes;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(size_t off, size_t bytes)
+    {
+        bytes = round_up(bytes, kAlign);
+        used_ -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto next = std::next(it);
+        if (next != free_.end() && it->first + it->second == next->first) {
+            it->second += next->second;
+            free_.erase(next);
+        }
+        if (it != free_.begin()) {
+            auto prev = std::prev(it);
+            if (prev->first + prev->second == it->first) {
+                prev->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    size_t used() const { return used_; }
+    size_t capacity() const { return cap_; }
+
+  private:
+    size_t cap_ = 0;
+    size_t used_ = 0;
+    std::map<size_t, size_t> free_;
+};
+
+struct Slot {
+    bool used = false;
+    int id = -1;
+    size_t off = 0;    // arena offset of U (U* follows)
+    size_t bytes = 0;  // U + U*
+    mhd_hip_instance desc{};
+    bool finished = false;
+    mhd_hip_result result{};
+    int bad_dt = 0;
+};
+
+size_t state_elems(int nx, int ny)
+{
+    return static_cast<size_t>(nx + 4) * static_cast<size_t>(ny + 4);
+}
+
+}  // namespace
+
+struct mhd_hip_batch {
+    static constexpr int kSteps = 8;
+
+    int device = -1;
+    int precision = MHD_HIP_FP64;
+    size_t elem = 8;
+    int limiter = MHD_HIP_MC;
+    bool nonideal = false;
+    mhd_hip_params phys_template{};
+
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    void* arena_base = nullptr;
+    Arena arena;
+    int max_instances = 0;
+    std::vector<Slot> slots;
+    int next_id = 0;
+
+    BatchInst* d_insts = nullptr;
+    Ctrl* d_ctrl = nullptr;
+    BatchTile* d_tiles = nullptr;
+    int* d_ntiles = nullptr;
+    size_t tile_cap = 0;
+
+    BatchInst* h_insts = nullptr;  // pinned staging
+    Ctrl* h_ctrl = nullptr;
+    BatchTile* h_tiles = nullptr;
+    int* h_ntiles = nullptr;
+    size_t h_tile_cap = 0;
+    double* h_stage = nullptr;  // pinned staging for state upload/download
+    size_t h_stage_bytes = 0;
+
+    bool dirty = true;
+    int seg_rows = 0;  // 0: autotune on the first advance
+    int tuned_rows = 0;
+    int ntiles = 0;
+    int blocks = 0;
+    double last_seconds = 0.0;
+    hipGraphExec_t graph = nullptr;
+
+    ~mhd_hip_batch()
+    {
+        if (device >= 0) hipSetDevice(device);
+        if (stream) hipStreamSynchronize(stream);
+        if (graph) hipGraphExecDestroy(graph);
+        if (arena_base) hipFree(arena_base);
+        if (d_insts) hipFree(d_insts);
+        if (d_ctrl) hipFree(d_ctrl);
+        if (d_tiles) hipFree(d_tiles);
+        if (d_ntiles) hipFree(d_ntiles);
+        if (h_insts) hipHostFree(h_insts);
+        if (h_ctrl) hipHostFree(h_ctrl);
+        if (h_tiles) hipHostFree(h_tiles);
+        if (h_ntiles) hipHostFree(h_ntiles);
+        if (h_stage) hipHostFree(h_stage);
+        if (ev0) hipEventDestroy(ev0);
+        if (ev1) hipEventDestroy(ev1);
+        if (stream) hipStreamDestroy(stream);
+    }
+
+    void init(int prec, const mhd_hip_params& tmpl, size_t budget, int max_inst)
+    {
+        if (prec != MHD_HIP_FP64 && prec != MHD_HIP_FP32) {
+            throw InvalidArgument("precision must be MHD_HIP_FP64 or MHD_HIP_FP32");
+        }
+        if (tmpl.limiter != MHD_HIP_MINMOD && tmpl.limiter != MHD_HIP_MC) {
+            throw InvalidArgument("unknown slope limiter");
+        }
+        if (max_inst < 1) {
+            throw InvalidArgument("max_instances must be positive");
+        }
+        if (budget < kAlign) {
+            throw InvalidArgument("HBM budget too small");
+        }
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count < 1) {
+            throw NoDevice("no HIP device available");
+        }
+        check(hipGetDevice(&device), "hipGetDevice");
+        hipDeviceProp_t prop;
+        check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+            throw NoDevice("current HIP device is not gfx950 (MI355X)");
+        }
+        precision = prec;
+        elem = prec == MHD_HIP_FP64 ? 8 : 4;
+        phys_template = tmpl;
+        limiter = tmpl.limiter;
+        // include/physics_config/mhd_config.hpp:90-94
+        nonideal = (tmpl.ohmic && tmpl.eta_ohm != 0.0) || (tmpl.hall && tmpl.eta_hall != 0.0) ||
+                   (tmpl.ambipolar && tmpl.eta_ambipolar != 0.0);
+        max_instances = max_inst;
+        slots.assign(max_inst, Slot{});
+        check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+        check(hipEventCreate(&ev0), "hipEventCreate");
+        check(hipEventCreate(&ev1), "hipEventCreate");
+        budget = round_up(budget, kAlign);
+        check(hipMalloc(&arena_base, budget), "hipMalloc(arena)");
+        arena.reset(budget);
+        check(hipMalloc(reinterpret_cast<void**>(&d_insts), sizeof(BatchInst) * max_inst),
+              "hipMalloc(insts)");
+        check(hipMalloc(reinterpret_cast<void**>(&d_ctrl), sizeof(Ctrl) * max_inst),
+              "hipMalloc(ctrl)");
+        check(hipMalloc(reinterpret_cast<void**>(&d_ntiles), sizeof(int)), "hipMalloc(ntiles)");
+        check(hipHostMalloc(reinterpret_cast<void**>(&h_insts), sizeof(BatchInst) * max_inst, 0),
+              "hipHostMalloc");
+        check(hipHostMalloc(reinterpret_cast<void**>(&h_ctrl), sizeof(Ctrl) * max_inst, 0),
+              "hipHostMalloc");
+        check(hipHostMalloc(reinterpret_cast<void**>(&h_ntiles), sizeof(int), 0), "hipHostMalloc");
+        std::memset(h_insts, 0, sizeof(BatchInst) * max_inst);
+        std::memset(h_ctrl, 0, sizeof(Ctrl) * max_inst);
+        *h_ntiles = 0;
+        check(hipMemcpyAsync(d_insts, h_insts, sizeof(BatchInst) * max_inst,
+                             hipMemcpyHostToDevice, stream),
+              "hipMemcpyAsync");
+        check(hipMemcpyAsync(d_ntiles, h_ntiles, sizeof(int), hipMemcpyHostToDevice, stream),
+              "hipMemcpyAsync");
+        ensure_tiles(1024);
+        const int per_cu = precision == MHD_HIP_FP64
+                               ? mhdhip::batch_blocks_per_cu<double>(limiter, nonideal)
+                               : mhdhip::batch_blocks_per_cu<float>(limiter, nonideal);
+        blocks = std::max(1, per_cu) * prop.multiProcessorCount;
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    }
+
+    void ensure_tiles(size_t n)
+    {
+        if (n <= tile_cap) {
+            return;
+        }
+        const size_t cap = std::max(n, tile_cap * 2);
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        if (d_tiles) hipFree(d_tiles);
+        if (h_tiles) hipHostFree(h_tiles);
+        check(hipMalloc(reinterpret_cast<void**>(&d_tiles), sizeof(BatchTile) * cap),
+              "hipMalloc(tiles)");
+        check(hipHostMalloc(reinterpret_cast<void**>(&h_tiles), sizeof(BatchTile) * cap, 0),
+              "hipHostMalloc(tiles)");
+        tile_cap = cap;
+        h_tile_cap = cap;
+        if (graph) {
+            hipGraphExecDestroy(graph);
+            graph = nullptr;
+        }
+        dirty = true;
+    }
+
+    void ensure_stage(size_t bytes)
+    {
+        if (bytes <= h_stage_bytes) {
+            return;
+        }
+        if (h_stage) hipHostFree(h_stage);
+        h_stage = nullptr;
+        check(hipHostMalloc(reinterpret_cast<void**>(&h_stage), bytes, 0), "hipHostMalloc(stage)");
+        h_stage_bytes = bytes;
+    }
+
+    static size_t instance_bytes(int prec, int nx, int ny)
+    {
+        const size_t e = prec == MHD_HIP_FP64 ? 8 : 4;
+        return 2 * round_up(9 * state_elems(nx, ny) * e, kAlign);
+    }
+
+    Slot& slot_of(int id)
+    {
+        for (auto& s : slots) {
+            if (s.used && s.id == id) {
+                return s;
+            }
+        }
+        throw InvalidArgument("unknown instance id " + std::to_string(id));
+    }
+
+    int slot_index(const Slot& s) const { return static_cast<int>(&s - slots.data()); }
+
+    Physics physics_of(const mhd_hip_instance& d) const
+    {
+        const double dx = (d.x_max - d.x_min) / static_cast<double>(d.nx);
+        const double dy = (d.y_max - d.y_min) / static_cast<double>(d.ny);
+        Physics ph{};
+        ph.gamma = d.gamma;
+        ph.gm1 = d.gamma - 1.0;
+        ph.inv_gm1 = 1.0 / (d.gamma - 1.0);
+        ph.cfl = d.cfl;
+        ph.inv_dx = 1.0 / dx;
+        ph.inv_dy = 1.0 / dy;
+        ph.inv_2dx = 0.5 / dx;
+        ph.inv_2dy = 0.5 / dy;
+        ph.glm_alpha = d.glm_alpha;
+        ph.h = (dy < dx) ? dy : dx;
+        const mhd_hip_params& t = phys_template;
+        double eta = 0.0;
+        auto mx = [](double a, double b) { return (a < b) ? b : a; };
+        if (t.ohmic) eta = mx(eta, std::fabs(t.eta_ohm));
+        if (t.hall) eta = mx(eta, std::fabs(t.eta_hall));
+        if (t.ambipolar) eta = mx(eta, std::fabs(t.eta_ambipolar));
+        ph.has_diff = eta > 0.0 ? 1 : 0;
+        ph.dt_diff = ph.has_diff ? d.cfl * ph.h * ph.h / (2.0 * 2.0 * eta) : 0.0;
+        ph.t_end = d.t_end;
+        ph.max_steps = d.max_steps;
+        ph.ni.ohmic = t.ohmic;
+        ph.ni.hall = t.hall;
+        ph.ni.ambipolar = t.ambipolar;
+        ph.ni.eta_ohm = t.eta_ohm;
+        ph.ni.eta_hall = t.eta_hall;
+        ph.ni.eta_ambipolar = t.eta_ambipolar;
+        return ph;
+    }
+
+    static Geom geom_of(const mhd_hip_instance& d)
+    {
+        Geom g{};
+        g.nx = d.nx;
+        g.ny = d.ny;
+        g.ng = 2;
+        g.nxt = d.nx + 4;
+        g.nyt = d.ny + 4;
+        g.plane = static_cast<long long>(g.nxt) * g.nyt;
+        g.bc_x = d.bc_x;
+        g.bc_y = d.bc_y;
+        return g;
+    }
+
+    template <typename R>
+    void upload_state(const Slot& s, const double* const comps[9])
+    {
+        const size_t n = state_elems(s.desc.nx, s.desc.ny);
+        ensure_stage(9 * n * sizeof(R));
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");  // staging reuse
+        R* st = reinterpret_cast<R*>(h_stage);
+        for (int c = 0; c < 9; ++c) {
+            for (size_t k = 0; k < n; ++k) {
+                st[c * n + k] = static_cast<R>(comps[c][k]);
+            }
+        }
+        char* base = static_cast<char*>(arena_base) + s.off;
+        check(hipMemcpyAsync(base, st, 9 * n * sizeof(R), hipMemcpyHostToDevice, stream),
+              "hipMemcpyAsync(upload)");
+    }
+
+    template <typename R>
+    void start_instance(int si)
+    {
+        const Slot& s = slots[si];
+        const Geom g = geom_of(s.desc);
+        const Physics ph = physics_of(s.desc);
+        R* u = reinterpret_cast<R*>(static_cast<char*>(arena_base) + s.off);
+        check(mhdhip::launch_ctrl_init<R>(d_ctrl + si, stream), "ctrl_init");
+        check(mhdhip::launch_cfl<R>(u, g, ph, nullptr, d_ctrl + si, stream), "cfl");
+    }
+
+    int add(const mhd_hip_instance& d, const double* const comps[9])
+    {
+        if (d.nx < 1 || d.ny < 1) {
+            throw InvalidArgument("instance grid must have at least one interior cell per axis");
+        }
+        if (d.t_end < 0.0) {
+            throw InvalidArgument("t_end must be non-negative");
+        }
+        if (d.cfl <= 0.0) {
+            throw InvalidArgument("CFL number must be positive");
+        }
+        if (d.max_steps < 0) {
+            throw InvalidArgument("max_steps must be non-negative");
+        }
+        auto valid_bc = [](int b) { return b == MHD_HIP_PERIODIC || b == MHD_HIP_OUTFLOW; };
+        if (!valid_bc(d.bc_x) || !valid_bc(d.bc_y)) {
+            throw InvalidArgument("instance boundaries must be periodic or outflow");
+        }
+        int si = -1;
+        for (int k = 0; k < max_instances; ++k) {
+            if (!slots[k].used) {
+                si = k;
+                break;
+            }
+        }
+        if (si < 0) {
+            throw Full("all instance slots are in use");
+        }
+        const size_t bytes = instance_bytes(precision, d.nx, d.ny);
+        size_t off = 0;
+        if (!arena.alloc(bytes, off)) {
+            throw Full("HBM budget exhausted (" + std::to_string(arena.used()) + " of " +
+                       std::to_string(arena.capacity()) + " bytes in use)");
+        }
+        hipSetDevice(device);
+        Slot& s = slots[si];
+        s = Slot{};
+        s.used = true;
+        s.id = next_id++;
+        s.off = off;
+        s.bytes = bytes;
+        s.desc = d;
+        try {
+            if (precision == MHD_HIP_FP64) {
+                upload_state<double>(s, comps);
+                start_instance<double>(si);
+            } else {
+                upload_state<float>(s, comps);
+                start_instance<float>(si);
+            }
+        } catch (...) {
+            arena.release(off, bytes);
+            s = Slot{};
+            throw;
+        }
+        dirty = true;
+        return s.id;
+    }
+
+    void remove(int id)
+    {
+        Slot& s = slot_of(id);
+        hipSetDevice(device);
+        // in-flight replays may still read this slot's state: release only after them
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        arena.release(s.off, s.bytes);
+        s = Slot{};
+        dirty = true;
+    }
+
+    int rows_for_tiles() const
+    {
+        if (seg_rows > 0) return seg_rows;
+        if (tuned_rows > 0) return tuned_rows;
+        return 32;
+    }
+
+    void build_tables(int rows)
+    {
+        size_t n = 0;
+        for (const auto& s : slots) {
+            if (s.used && !s.finished) {
+                const int strips = (s.desc.nx + mhdhip::kStripCells - 1) / mhdhip::kStripCells;
+                const int segs = (s.desc.ny + rows - 1) / rows;
+                n += static_cast<size_t>(strips) * segs;
+            }
+        }
+        ensure_tiles(std::max<size_t>(n, 1));
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");  // staging reuse
+        size_t k = 0;
+        for (int si = 0; si < max_instances; ++si) {
+            const Slot& s = slots[si];
+            BatchInst& in = h_insts[si];
+            std::memset(&in, 0, sizeof(in));
+            if (!s.used) {
+                continue;
+            }
+            char* base = static_cast<char*>(arena_base) + s.off;
+            in.U = base;
+            in.Us = base + s.bytes / 2;
+            in.g = geom_of(s.desc);
+            in.ph = physics_of(s.desc);
+            in.enabled = 1;
+            if (s.finished) {
+                continue;
+            }
+            const int strips = (s.desc.nx + mhdhip::kStripCells - 1) / mhdhip::kStripCells;
+            for (int r0 = 2; r0 < 2 + s.desc.ny; r0 += rows) {
+                for (int st = 0; st < strips; ++st) {
+                    h_tiles[k++] = BatchTile{si, st, r0, std::min(r0 + rows, 2 + s.desc.ny)};
+                }
+            }
+        }
+        ntiles = static_cast<int>(k);
+        *h_ntiles = ntiles;
+        check(hipMemcpyAsync(d_insts, h_insts, sizeof(BatchInst) * max_instances,
+                             hipMemcpyHostToDevice, stream),
+              "hipMemcpyAsync(insts)");
+        if (k > 0) {
+            check(hipMemcpyAsync(d_tiles, h_tiles, sizeof(BatchTile) * k, hipMemcpyHostToDevice,
+                                 stream),
+                  "hipMemcpyAsync(tiles)");
+        }
+        check(hipMemcpyAsync(d_ntiles, h_ntiles, sizeof(int), hipMemcpyHostToDevice, stream),
+              "hipMemcpyAsync(ntiles)");
+    }
+
+    BatchTables tables() const { return BatchTables{d_insts, d_ctrl, d_tiles, d_ntiles}; }
+
+    template <typename R>
+    void ensure_graph()
+    {
+        if (graph) {
+            return;
+        }
+        hipGraph_t g = nullptr;
+        check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "BeginCapture");
+        try {
+            for (int k = 0; k < kSteps; ++k) {
+                const int p = k & 1;
+                check(mhdhip::launch_batch_stage<R>(1, limiter, nonideal, tables(), p, blocks,
+                                                    stream),
+                      "batch stage1");
+                check(mhdhip::launch_batch_stage<R>(2, limiter, nonideal, tables(), p, blocks,
+                                                    stream),
+                      "batch stage2");
+            }
+        } catch (...) {
+            hipStreamEndCapture(stream, &g);
+            if (g) hipGraphDestroy(g);
+            throw;
+        }
+        check(hipStreamEndCapture(stream, &g), "EndCapture");
+        const hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        check(e, "hipGraphInstantiate");
+    }
+
+    // Autotuner: times one stage-1 pass (which leaves U untouched) over the current instance
+    // set for candidate row-segment lengths and keeps the fastest.
+    template <typename R>
+    void autotune()
+    {
+        int best = 32;
+        float best_ms = 1e30f;
+        for (int rows : {8, 16, 32, 64, 128}) {
+            build_tables(rows);
+            check(mhdhip::launch_batch_stage<R>(1, limiter, nonideal, tables(), 0, blocks, stream),
+                  "tune");
+            float ms_best = 1e30f;
+            for (int rep = 0; rep < 2; ++rep) {
+                check(hipEventRecord(ev0, stream), "hipEventRecord");
+                check(mhdhip::launch_batch_stage<R>(1, limiter, nonideal, tables(), 0, blocks,
+                                                    stream),
+                      "tune");
+                check(hipEventRecord(ev1, stream), "hipEventRecord");
+                check(hipEventSynchronize(ev1), "hipEventSynchronize");
+                float ms = 0.0f;
+                check(hipEventElapsedTime(&ms, ev0, ev1), "hipEventElapsedTime");
+                ms_best = std::min(ms_best, ms);
+            }
+            if (ms_best < best_ms) {
+                best_ms = ms_best;
+                best = rows;
+            }
+        }
+        tuned_rows = best;
+        dirty = true;
+    }
+
+    void read_ctrl()
+    {
+        check(hipMemcpyAsync(h_ctrl, d_ctrl, sizeof(Ctrl) * max_instances, hipMemcpyDeviceToHost,
+                             stream),
+              "hipMemcpyAsync(ctrl)");
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        for (int si = 0; si < max_instances; ++si) {
+            Slot& s = slots[si];
+            if (!s.used || s.finished) {
+                continue;
+            }
+            const Ctrl& c = h_ctrl[si];
+            // replays hold an even number of steps: the current slot is parity 0
+            s.result.t = c.t[0];
+            s.result.steps = c.steps[0];
+            s.result.c_h = c.ch_last[0];
+            s.bad_dt = c.bad_dt;
+            if (c.bad_dt || !(c.t[0] < s.desc.t_end && c.steps[0] < s.desc.max_steps)) {
+                s.finished = true;
+                dirty = true;  // drop its tiles from the next replay
+            }
+        }
+    }
+
+    template <typename R>
+    int advance_t(int max_replays)
+    {
+        hipSetDevice(device);
+        check(hipEventRecord(ev0, stream), "hipEventRecord");
+        int active = 0;
+        for (int rep = 0; max_replays < 0 || rep < max_replays; ++rep) {
+            read_ctrl();
+            active = 0;
+            for (const auto& s : slots) {
+                active += (s.used && !s.finished) ? 1 : 0;
+            }
+            if (active == 0) {
+                break;
+            }
+            if (seg_rows == 0 && tuned_rows == 0) {
+                autotune<R>();
+            }
+            if (dirty) {
+                build_tables(rows_for_tiles());
+                dirty = false;
+            }
+            ensure_graph<R>();
+            check(hipGraphLaunch(graph, stream), "hipGraphLaunch");
+        }
+        check(hipEventRecord(ev1, stream), "hipEventRecord");
+        check(hipEventSynchronize(ev1), "hipEventSynchronize");
+        float ms = 0.0f;
+        check(hipEventElapsedTime(&ms, ev0, ev1), "hipEventElapsedTime");
+        last_seconds = ms * 1e-3;
+        read_ctrl();
+        active = 0;
+        for (const auto& s : slots) {
+            active += (s.used && !s.finished) ? 1 : 0;
+        }
+        return active;
+    }
+
+    int advance(int max_replays)
+    {
+        return precision == MHD_HIP_FP64 ? advance_t<double>(max_replays)
+                                         : advance_t<float>(max_replays);
+    }
+
+    template <typename R>
+    void download_t(const Slot& s, double* const comps[9])
+    {
+        const Geom g = geom_of(s.desc);
+        R* u = reinterpret_cast<R*>(static_cast<char*>(arena_base) + s.off);
+        const Gate open{d_ctrl, 0.0, 0, -1};
+        // include/mhd/time_integrator.hpp:101-102: the final ghost fill of a solve
+        check(mhdhip::launch_ghost_fill<R>(u, g, open, stream), "ghost_fill");
+        const size_t n = state_elems(s.desc.nx, s.desc.ny);
+        ensure_stage(9 * n * sizeof(R));
+        check(hipMemcpyAsync(h_stage, u, 9 * n * sizeof(R), hipMemcpyDeviceToHost, stream),
+              "hipMemcpyAsync(download)");
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        const R* st = reinterpret_cast<const R*>(h_stage);
+        for (int c = 0; c < 9; ++c) {
+            for (size_t k = 0; k < n; ++k) {
+                comps[c][k] = static_cast<double>(st[c * n + k]);
+            }
+        }
+    }
+
+    void download(int id, double* const comps[9], size_t n)
+    {
+        const Slot& s = slot_of(id);
+        if (n != state_elems(s.desc.nx, s.desc.ny)) {
+            throw InvalidArgument("download: size does not match the instance's full layout");
+        }
+        hipSetDevice(device);
+        if (precision == MHD_HIP_FP64) {
+            download_t<double>(s, comps);
+        } else {
+            download_t<float>(s, comps);
+        }
+    }
+};
+
+extern "C" {
+
+size_t mhd_hip_batch_instance_bytes(int precision, int nx, int ny)
+{
+    if (nx < 1 || ny < 1 || (precision != MHD_HIP_FP64 && precision != MHD_HIP_FP32)) {
+        return 0;
+    }
+    return mhd_hip_batch::instance_bytes(precision, nx, ny);
+}
+
+int mhd_hip_batch_create(int precision, const mhd_hip_params* physics, size_t hbm_budget_bytes,
+                         int max_instances, mhd_hip_batch** out)
+{
+    return guarded([&] {
+        if (!physics || !out) throw InvalidArgument("null argument");
+        *out = nullptr;
+        auto b = std::make_unique<mhd_hip_batch>();
+        b->init(precision, *physics, hbm_budget_bytes, max_instances);
+        *out = b.release();
+    });
+}
+
+void mhd_hip_batch_destroy(mhd_hip_batch* b)
+{
+    delete b;
+}
+
+int mhd_hip_batch_add(mhd_hip_batch* b, const mhd_hip_instance* inst,
+                      const double* const comps[9], int* id)
+{
+    return guarded([&] {
+        if (!b || !inst || !comps || !id) throw InvalidArgument("null argument");
+        for (int c = 0; c < 9; ++c) {
+            if (!comps[c]) throw InvalidArgument("null component pointer");
+        }
+        *id = b->add(*inst, comps);
+    });
+}
+
+int mhd_hip_batch_remove(mhd_hip_batch* b, int id)
+{
+    return guarded([&] {
+        if (!b) throw InvalidArgument("null batch");
+        b->remove(id);
+    });
+}
+
+int mhd_hip_batch_advance(mhd_hip_batch* b, int max_replays, int* active)
+{
+    return guarded([&] {
+        if (!b || !active) throw InvalidArgument("null argument");
+        *active = b->advance(max_replays);
+    });
+}
+
+int mhd_hip_batch_result(mhd_hip_batch* b, int id, mhd_hip_result* result, int* finished)
+{
+    return guarded([&] {
+        if (!b || !result || !finished) throw InvalidArgument("null argument");
+        const Slot& s = b->slot_of(id);
+        *result = s.result;
+        *finished = s.finished ? 1 : 0;
+        if (s.bad_dt) {
+            throw InvalidArgument("ssp_rk2_step requires positive dt");
+        }
+        if (s.finished && s.result.t < s.desc.t_end && s.result.steps >= s.desc.max_steps &&
+            s.desc.t_end != std::numeric_limits<double>::infinity()) {
+            throw std::runtime_error("solve: reached max_steps before t_end");
+        }
+    });
+}
+
+int mhd_hip_batch_download(mhd_hip_batch* b, int id, double* const comps[9], size_t n)
+{
+    return guarded([&] {
+        if (!b || !comps) throw InvalidArgument("null argument");
+        for (int c = 0; c < 9; ++c) {
+            if (!comps[c]) throw InvalidArgument("null component pointer");
+        }
+        b->download(id, comps, n);
+    });
+}
+
+int mhd_hip_batch_set_seg_rows(mhd_hip_batch* b, int rows)
+{
+    return guarded([&] {
+        if (!b) throw InvalidArgument("null batch");
+        if (rows < 0) throw InvalidArgument("rows per wave segment must be >= 0");
+        b->seg_rows = rows;
+        b->dirty = true;
+    });
+}
+
+int mhd_hip_batch_info(const mhd_hip_batch* b, mhd_hip_batch_stats* st)
+{
+    return guarded([&] {
+        if (!b || !st) throw InvalidArgument("null argument");
+        int n = 0, act = 0;
+        for (const auto& s : b->slots) {
+            n += s.used ? 1 : 0;
+            act += (s.used && !s.finished) ? 1 : 0;
+        }
+        st->instances = n;
+        st->active = act;
+        st->tiles = b->ntiles;
+        st->seg_rows = b->seg_rows > 0 ? b->seg_rows : b->tuned_rows;
+        st->blocks = b->blocks;
+        st->bytes_used = b->arena.used();
+        st->bytes_budget = b->arena.capacity();
+        st->last_seconds = b->last_seconds;
+    });
+}
+
+}  // extern "C"

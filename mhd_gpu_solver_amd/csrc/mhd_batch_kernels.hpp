@@ -1,0 +1,86 @@
+// Batched stage kernel: many independent grids of different sizes advance in one persistent
+// launch per RK stage, each with its own dt, gamma, CFL number, boundary types and end time.
+//
+// The work list is a device table of tiles (instance, strip, row range); every wave of the
+// persistent grid walks the list with a stride of the total wave count, after the XCD-aware
+// remap of its workgroup, so neighbouring tiles meet in one L2. The kernel's arguments are
+// table pointers only: the host may add and remove instances between launches (by copies on
+// the same stream) and a captured graph stays valid.
+#pragma once
+
+#include "mhd_kernels.hpp"
+
+namespace mhdhip {
+
+struct BatchInst {
+    void* U;       // this instance's 9 planes of U
+    void* Us;      // and of U*
+    Geom g;
+    Physics ph;
+    int enabled;
+    int pad;
+};
+
+struct BatchTile {
+    int inst;
+    int strip;
+    int r0, r1;  // global rows [r0, r1) marched by one wave
+};
+
+struct BatchTables {
+    const BatchInst* insts;
+    Ctrl* ctrl;  // one control block per instance slot
+    const BatchTile* tiles;
+    const int* ntiles;
+};
+
+template <int STAGE, int LIM, int NI, typename R>
+__global__ __launch_bounds__(kBlock) void k_batch_stage(BatchTables tb, int parity)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x >> 6;
+    const int gw = xcd_logical_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
+    const int nw = gridDim.x * kWavesPerBlock;
+    const int n = *tb.ntiles;
+    __shared__ R ring[kWavesPerBlock][3][kNC][kWave];
+    for (int t = gw; t < n; t += nw) {
+        const BatchTile tile = tb.tiles[t];
+        const BatchInst& in = tb.insts[tile.inst];
+        if (!in.enabled) {
+            continue;
+        }
+        Ctrl* ctrl = tb.ctrl + tile.inst;
+        const StepState st = step_state(ctrl, in.ph, parity);
+        // the instance's first tile keeps its step bookkeeping (as block 0 does in k_stage)
+        const bool lead = tile.strip == 0 && tile.r0 == in.g.ng && lane == 0;
+        if (STAGE == 1 && lead) {
+            ctrl->chmax[parity ^ 1] = 0ull;
+            if (!st.active && ctrl_pending(ctrl, in.ph, parity)) {
+                ctrl->bad_dt = 1;
+            }
+        }
+        if (STAGE == 2 && lead) {
+            const int q = parity ^ 1;
+            ctrl->t[q] = st.active ? st.t + st.dt : st.t;
+            ctrl->steps[q] = st.active ? st.steps + 1 : st.steps;
+            ctrl->ch_last[q] = st.active ? st.c_h : ctrl->ch_last[parity];
+        }
+        if (!st.active) {
+            continue;
+        }
+        const R* src = static_cast<const R*>(STAGE == 1 ? in.U : in.Us);
+        const R* u0 = static_cast<const R*>(in.U);
+        R* dst = static_cast<R*>(STAGE == 1 ? in.Us : in.U);
+        const R local = march_tile<STAGE, LIM, NI, 1, R>(src, u0, dst, nullptr, in.g, in.ph, st,
+                                                         tile.strip, tile.r0, tile.r1, ring[wave],
+                                                         lane);
+        if (STAGE == 2) {
+            const R m = wave_max(local);
+            if (lane == 0) {
+                atomic_max_speed(&ctrl->chmax[parity ^ 1], static_cast<double>(m));
+            }
+        }
+    }
+}
+
+}  // namespace mhdhip

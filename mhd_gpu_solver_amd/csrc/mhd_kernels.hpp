@@ -319,47 +319,15 @@ struct StageBuffers {
     int dt_cap;
 };
 
+// March rows [r0, r1) of one 60-column strip (see the comment above StageBuffers) and apply the
+// RK stage to its cells. `ring` is this wave's LDS slice. Returns the wave-local maximum CFL
+// signal speed of the updated fluid cells (stage 2; 0 in stage 1).
 template <int STAGE, int LIM, int NI, int REMAP, typename R>
-__global__ __launch_bounds__(kBlock) void k_stage(StageBuffers b, Geom g, Physics ph, Tiling tl,
-                                                  int parity)
+__device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, R* dst,
+                                        const uint8_t* mask, const Geom& g, const Physics& ph,
+                                        const StepState& st, int strip, int r0, int r1,
+                                        R (&ring)[3][kNC][kWave], int lane)
 {
-    const StepState st = step_state(b.ctrl, ph, parity);
-    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    if (STAGE == 1 && lead) {
-        b.ctrl->chmax[parity ^ 1] = 0ull;
-        if (!st.active && ctrl_pending(b.ctrl, ph, parity)) {
-            b.ctrl->bad_dt = 1;
-        }
-    }
-    if (STAGE == 2 && lead) {
-        const int q = parity ^ 1;
-        b.ctrl->t[q] = st.active ? st.t + st.dt : st.t;
-        b.ctrl->steps[q] = st.active ? st.steps + 1 : st.steps;
-        b.ctrl->ch_last[q] = st.active ? st.c_h : b.ctrl->ch_last[parity];
-        if (st.active && b.dt_log != nullptr && st.steps < b.dt_cap) {
-            b.dt_log[st.steps] = st.dt;
-        }
-    }
-    if (!st.active) {
-        return;
-    }
-    // block -> (strip, segment group); its four waves march four consecutive row segments.
-    // Consecutive logical blocks (neighbouring strips) are placed on the same XCD so that the
-    // halo columns and the cache lines shared across a strip edge meet in one L2.
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x >> 6;
-    const int logical = xcd_logical_block(blockIdx.x, gridDim.x);
-    const int strip = logical % tl.strips;
-    const int seg = (logical / tl.strips) * kWavesPerBlock + wave;
-    if (strip >= tl.strips || seg >= tl.segs) {
-        return;
-    }
-    const int r0 = g.ng + seg * tl.seg_rows;
-    const int r1 = min(r0 + tl.seg_rows, g.ng + g.ny);
-
-    const R* __restrict__ src = static_cast<const R*>(b.src);
-    const R* u0 = static_cast<const R*>(b.u0);
-    R* dst = static_cast<R*>(b.dst);
     const long long P = g.plane;
 
     const int gi = g.ng + strip * kStripCells - 2 + lane;
@@ -397,13 +365,10 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageBuffers b, Geom g, Physic
     R Jp[3] = {R(0), R(0), R(0)};
     R local_speed = R(0);
 
-    // wave-private ring of the last three raw (conserved) rows: the RK update of row c-1 reads
-    // its stage input from here instead of global memory
-    __shared__ R ring[kWavesPerBlock][3][kNC][kWave];
     auto ring_put = [&](int q, const R (&u)[kNC]) {
 #pragma unroll
         for (int k = 0; k < kNC; ++k) {
-            ring[wave][q % 3][k][lane] = u[k];
+            ring[q % 3][k][lane] = u[k];
         }
     };
 
@@ -497,8 +462,8 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageBuffers b, Geom g, Physic
                     L[PSI] -= psi_damp * Wa[PSI];
                 }
                 if (cell_lane) {
-                    if (b.mask != nullptr &&
-                        b.mask[static_cast<long long>(rr - g.ng) * g.nx + (gi - g.ng)] == 0) {
+                    if (mask != nullptr &&
+                        mask[static_cast<long long>(rr - g.ng) * g.nx + (gi - g.ng)] == 0) {
 #pragma unroll
                         for (int k = 0; k < kNC; ++k) {
                             L[k] = R(0);
@@ -510,14 +475,14 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageBuffers b, Geom g, Physic
                         // include/mhd/field_ops.hpp:47-53
 #pragma unroll
                         for (int k = 0; k < kNC; ++k) {
-                            dst[k * P + cell] = ring[wave][slot][k][lane] + dt * L[k];
+                            dst[k * P + cell] = ring[slot][k][lane] + dt * L[k];
                         }
                     } else {
                         // include/mhd/field_ops.hpp:55-63
                         R un[kNC];
 #pragma unroll
                         for (int k = 0; k < kNC; ++k) {
-                            un[k] = half * (u0[k * P + cell] + ring[wave][slot][k][lane]) +
+                            un[k] = half * (u0[k * P + cell] + ring[slot][k][lane]) +
                                     half_dt * L[k];
                         }
 #pragma unroll
@@ -525,8 +490,8 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageBuffers b, Geom g, Physic
                             dst[k * P + cell] = un[k];
                         }
                         const bool fluid =
-                            b.mask == nullptr ||
-                            b.mask[static_cast<long long>(rr - g.ng) * g.nx + (gi - g.ng)] != 0;
+                            mask == nullptr ||
+                            mask[static_cast<long long>(rr - g.ng) * g.nx + (gi - g.ng)] != 0;
                         if (fluid) {
                             local_speed = vmax(local_speed, cell_max_speed(un, gamma, gm1));
                         }
@@ -551,6 +516,55 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageBuffers b, Geom g, Physic
             }
         }
     }
+    return local_speed;
+}
+
+template <int STAGE, int LIM, int NI, int REMAP, typename R>
+__global__ __launch_bounds__(kBlock) void k_stage(StageBuffers b, Geom g, Physics ph, Tiling tl,
+                                                  int parity)
+{
+    const StepState st = step_state(b.ctrl, ph, parity);
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (STAGE == 1 && lead) {
+        b.ctrl->chmax[parity ^ 1] = 0ull;
+        if (!st.active && ctrl_pending(b.ctrl, ph, parity)) {
+            b.ctrl->bad_dt = 1;
+        }
+    }
+    if (STAGE == 2 && lead) {
+        const int q = parity ^ 1;
+        b.ctrl->t[q] = st.active ? st.t + st.dt : st.t;
+        b.ctrl->steps[q] = st.active ? st.steps + 1 : st.steps;
+        b.ctrl->ch_last[q] = st.active ? st.c_h : b.ctrl->ch_last[parity];
+        if (st.active && b.dt_log != nullptr && st.steps < b.dt_cap) {
+            b.dt_log[st.steps] = st.dt;
+        }
+    }
+    if (!st.active) {
+        return;
+    }
+    // block -> (strip, segment group); its four waves march four consecutive row segments.
+    // Consecutive logical blocks (neighbouring strips) are placed on the same XCD so that the
+    // halo columns and the cache lines shared across a strip edge meet in one L2.
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x >> 6;
+    const int logical = xcd_logical_block(blockIdx.x, gridDim.x);
+    const int strip = logical % tl.strips;
+    const int seg = (logical / tl.strips) * kWavesPerBlock + wave;
+    if (strip >= tl.strips || seg >= tl.segs) {
+        return;
+    }
+    const int r0 = g.ng + seg * tl.seg_rows;
+    const int r1 = min(r0 + tl.seg_rows, g.ng + g.ny);
+
+    const R* __restrict__ src = static_cast<const R*>(b.src);
+    const R* u0 = static_cast<const R*>(b.u0);
+    R* dst = static_cast<R*>(b.dst);
+    // wave-private ring of the last three raw (conserved) rows: the RK update of row c-1 reads
+    // its stage input from here instead of global memory
+    __shared__ R ring[kWavesPerBlock][3][kNC][kWave];
+    const R local_speed = march_tile<STAGE, LIM, NI, REMAP, R>(src, u0, dst, b.mask, g, ph, st,
+                                                               strip, r0, r1, ring[wave], lane);
     if (STAGE == 2) {
         const R m = wave_max(local_speed);
         if (lane == 0) {

@@ -2,7 +2,7 @@
 // mhd_fp64.hip (double, compiled with -ffp-contract=off) and mhd_fp32.hip (float, contraction on).
 #pragma once
 
-#include "mhd_kernels.hpp"
+#include "mhd_batch_kernels.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -30,5 +30,14 @@ template <typename R>
 hipError_t launch_stage(int stage, int limiter, bool nonideal, bool remap, const StageBuffers& b,
                         const Geom& g, const Physics& ph, const Tiling& tl, int parity,
                         hipStream_t s);
+
+// Batched engine: one persistent launch of `blocks` workgroups per stage.
+template <typename R>
+hipError_t launch_batch_stage(int stage, int limiter, bool nonideal, const BatchTables& tb,
+                              int parity, int blocks, hipStream_t s);
+
+// Workgroups of k_batch_stage that fit on one CU at once (occupancy query).
+template <typename R>
+int batch_blocks_per_cu(int limiter, bool nonideal);
 
 }  // namespace mhdhip

@@ -98,6 +98,57 @@ hipError_t launch_stage(int stage, int limiter, bool nonideal, bool remap, const
                : detail::launch_ni_remap<2, kMC, R>(nonideal, remap, b, g, ph, tl, parity, s);
 }
 
+namespace detail {
+
+template <int STAGE, int LIM, int NI, typename R>
+hipError_t launch_batch_one(const BatchTables& tb, int parity, int blocks, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_batch_stage<STAGE, LIM, NI, R>), dim3(blocks), dim3(kBlock), 0, s, tb,
+                       parity);
+    return hipGetLastError();
+}
+
+template <int STAGE, typename R>
+hipError_t launch_batch_lim(int limiter, bool ni, const BatchTables& tb, int parity, int blocks,
+                            hipStream_t s)
+{
+    if (limiter == kMinmod) {
+        return ni ? launch_batch_one<STAGE, kMinmod, 1, R>(tb, parity, blocks, s)
+                  : launch_batch_one<STAGE, kMinmod, 0, R>(tb, parity, blocks, s);
+    }
+    return ni ? launch_batch_one<STAGE, kMC, 1, R>(tb, parity, blocks, s)
+              : launch_batch_one<STAGE, kMC, 0, R>(tb, parity, blocks, s);
+}
+
+}  // namespace detail
+
+template <typename R>
+hipError_t launch_batch_stage(int stage, int limiter, bool nonideal, const BatchTables& tb,
+                              int parity, int blocks, hipStream_t s)
+{
+    return stage == 1 ? detail::launch_batch_lim<1, R>(limiter, nonideal, tb, parity, blocks, s)
+                      : detail::launch_batch_lim<2, R>(limiter, nonideal, tb, parity, blocks, s);
+}
+
+template <typename R>
+int batch_blocks_per_cu(int limiter, bool nonideal)
+{
+    int n = 0;
+    hipError_t e;
+    if (limiter == kMinmod) {
+        e = nonideal ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                           &n, k_batch_stage<2, kMinmod, 1, R>, kBlock, 0)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                           &n, k_batch_stage<2, kMinmod, 0, R>, kBlock, 0);
+    } else {
+        e = nonideal ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                           &n, k_batch_stage<2, kMC, 1, R>, kBlock, 0)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                           &n, k_batch_stage<2, kMC, 0, R>, kBlock, 0);
+    }
+    return (e == hipSuccess && n > 0) ? n : 1;
+}
+
 #define MHDHIP_INSTANTIATE(R)                                                                    \
     template hipError_t launch_ctrl_init<R>(Ctrl*, hipStream_t);                                 \
     template hipError_t launch_cfl<R>(const R*, const Geom&, const Physics&, const uint8_t*,     \
@@ -106,6 +157,9 @@ hipError_t launch_stage(int stage, int limiter, bool nonideal, bool remap, const
     template hipError_t launch_wall_fill<R>(R*, const Geom&, const long long*, int, double,      \
                                             const Gate&, hipStream_t);                           \
     template hipError_t launch_stage<R>(int, int, bool, bool, const StageBuffers&, const Geom&,  \
-                                        const Physics&, const Tiling&, int, hipStream_t);
+                                        const Physics&, const Tiling&, int, hipStream_t);         \
+    template hipError_t launch_batch_stage<R>(int, int, bool, const BatchTables&, int, int,       \
+                                              hipStream_t);                                       \
+    template int batch_blocks_per_cu<R>(int, bool);
 
 }  // namespace mhdhip

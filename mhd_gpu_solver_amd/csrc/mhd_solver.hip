@@ -6,6 +6,7 @@
 // (kSteps steps per replay, parity baked in), checking the device control block once per
 // replay while the next replay is already queued. Surplus steps after t_end are no-ops inside
 // the kernels, so the host never needs the step count in advance.
+#include "mhd_abi.hpp"
 #include "mhd_hip.h"
 #include "mhd_host.hpp"
 #include "mhd_launch.hpp"
@@ -26,9 +27,22 @@
 #include <string>
 #include <vector>
 
+namespace mhdhip {
+
+std::string& last_error_slot()
+{
+    thread_local std::string msg;
+    return msg;
+}
+
+}  // namespace mhdhip
+
 namespace {
 
-thread_local std::string g_last_error;
+using mhdhip::check;
+using mhdhip::guarded;
+using mhdhip::InvalidArgument;
+using mhdhip::NoDevice;
 
 // Process-wide autotuning results: best rows-per-wave for a (shape, precision, kernel variant).
 struct TuneKey {
@@ -41,44 +55,6 @@ struct TuneKey {
 };
 std::mutex g_tune_mutex;
 std::map<TuneKey, int> g_tuned;
-
-struct InvalidArgument : std::invalid_argument {
-    using std::invalid_argument::invalid_argument;
-};
-
-struct NoDevice : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-
-void check(hipError_t e, const char* what)
-{
-    if (e != hipSuccess) {
-        throw std::runtime_error(std::string("HIP error in ") + what + ": " +
-                                 hipGetErrorString(e));
-    }
-}
-
-template <class F>
-int guarded(F&& f)
-{
-    try {
-        f();
-        g_last_error.clear();
-        return MHD_HIP_OK;
-    } catch (const NoDevice& e) {
-        g_last_error = e.what();
-        return MHD_HIP_ENODEV;
-    } catch (const std::invalid_argument& e) {
-        g_last_error = e.what();
-        return MHD_HIP_EINVAL;
-    } catch (const std::exception& e) {
-        g_last_error = e.what();
-        return MHD_HIP_ERUNTIME;
-    } catch (...) {
-        g_last_error = "unknown error";
-        return MHD_HIP_ERUNTIME;
-    }
-}
 
 bool device_is_gfx950(int dev)
 {
@@ -778,7 +754,7 @@ int mhd_hip_available(void)
 
 const char* mhd_hip_last_error(void)
 {
-    return g_last_error.c_str();
+    return mhdhip::last_error_slot().c_str();
 }
 
 int mhd_hip_create(const mhd_hip_grid* grid, int precision, mhd_hip_solver** out)

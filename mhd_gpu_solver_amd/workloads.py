@@ -1,0 +1,74 @@
+"""Seeded synthetic workloads: the mixed-size ensemble and the request stream.
+
+All randomness comes from numpy's PCG64 generator (numpy.random.default_rng(seed)), so a seed
+reproduces the same instances on any machine.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+from .solver import init_ic
+
+B0_STD = 1.0 / math.sqrt(4.0 * math.pi)  # Orszag-Tang field normalisation B0 = 1/sqrt(4 pi)
+PERIODIC, OUTFLOW = 0, 2
+
+
+@dataclasses.dataclass
+class Spec:
+    """One solve: initial condition, grid, gamma, field scale, boundaries, end time."""
+    ic: str
+    n: int
+    gamma: float
+    b0: float
+    bc: tuple
+    t_end: float = math.inf
+    seed: int = 0
+    arrival: float = 0.0
+
+    def state(self):
+        return init_ic(self.ic, self.n, self.n, gamma=self.gamma, b0=self.b0, seed=self.seed)
+
+
+def ensemble(count=512, seed=2024, sizes=(64, 96, 128, 192, 256)):
+    """Independent Orszag-Tang / rotor instances on square grids drawn uniformly from `sizes`;
+    gamma ~ U[1.4, 5/3]; the field scale ~ U[0.5, 2] multiplies the Orszag-Tang B0 = 1/sqrt(4 pi)
+    or the rotor's Bx = 5/sqrt(4 pi). Orszag-Tang runs periodic, the rotor with outflow."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        ot = bool(rng.integers(0, 2) == 0)
+        n = int(rng.choice(sizes))
+        gamma = float(rng.uniform(1.4, 5.0 / 3.0))
+        scale = float(rng.uniform(0.5, 2.0))
+        if ot:
+            out.append(Spec("orszag_tang_b0", n, gamma, scale * B0_STD, (PERIODIC, PERIODIC)))
+        else:
+            out.append(Spec("rotor_b0", n, gamma, scale, (OUTFLOW, OUTFLOW)))
+    return out
+
+
+def request_stream(count, rate=200.0, seed=7, n_min=64, n_max=1024, t_min=0.05, t_max=0.5):
+    """Poisson arrivals (mean `rate` per second) of solves mixing the oblique circularly
+    polarised Alfven wave, the tanh Kelvin-Helmholtz layer (seeded vy noise) and the rotor;
+    square grids N ~ U{n_min..n_max}; t_end ~ U[t_min, t_max]."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for k in range(count):
+        t += float(rng.exponential(1.0 / rate))
+        kind = int(rng.integers(0, 3))
+        n = int(rng.integers(n_min, n_max + 1))
+        t_end = float(rng.uniform(t_min, t_max))
+        if kind == 0:
+            s = Spec("alfven_oblique", n, 5.0 / 3.0, 1.0, (PERIODIC, PERIODIC), t_end)
+        elif kind == 1:
+            s = Spec("kh_tanh", n, 5.0 / 3.0, 1.0, (PERIODIC, PERIODIC), t_end,
+                     seed=int(rng.integers(0, 2 ** 31)))
+        else:
+            s = Spec("rotor", n, 5.0 / 3.0, 1.0, (OUTFLOW, OUTFLOW), t_end)
+        s.arrival = t
+        out.append(s)
+    return out

@@ -1,0 +1,119 @@
+"""Batched engine: every instance gives the oracle's fp64 bits, its result is the same alone
+and in any mix (fp32 included), admission honours the HBM budget, and tiling does not matter."""
+import math
+
+import numpy as np
+import pytest
+
+from checkers import MC, MINMOD, OUTFLOW, PERIODIC, make_grid, make_params
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_specs(hip, count=9, seed=3):
+    from mhd_gpu_solver_amd import workloads
+    rng = np.random.default_rng(seed)
+    specs = []
+    for k in range(count):
+        n = int(rng.choice([17, 40, 61, 64, 96, 130]))
+        ot = k % 2 == 0
+        s = workloads.Spec("orszag_tang_b0" if ot else "rotor_b0", n,
+                           float(rng.uniform(1.4, 5 / 3)), float(rng.uniform(0.5, 2.0)),
+                           (PERIODIC, PERIODIC) if ot else (OUTFLOW, PERIODIC),
+                           t_end=float(rng.uniform(0.005, 0.02)))
+        specs.append(s)
+    return specs
+
+
+def oracle_run(oracle, spec, U0, limiter, cfl=0.4, steps=None):
+    g = make_grid(spec.n, spec.n)
+    p = make_params(t_end=spec.t_end if steps is None else 1e300, cfl=cfl, gamma=spec.gamma,
+                    limiter=limiter, max_steps=steps if steps is not None else 1_000_000)
+    return oracle.solve(g, U0, p, bc=spec.bc, allow_max_steps=steps is not None)
+
+
+def test_batch_fp64_matches_oracle_bitwise(hip, oracle):
+    specs = mixed_specs(hip)
+    eng = hip.BatchEngine("fp64", limiter=hip.SlopeLimiter.MC, hbm_budget=1 << 30)
+    ids = []
+    states = []
+    for s in specs:
+        U0 = s.state()
+        states.append(U0)
+        ids.append(eng.add(U0, s.t_end, gamma=s.gamma, bc=s.bc))
+    assert eng.advance() == 0
+    for s, i, U0 in zip(specs, ids, states):
+        r, fin = eng.result(i)
+        assert fin
+        Uo, info = oracle_run(oracle, s, U0, MC)
+        assert r.steps == info["steps"] and r.t == info["t"] and r.c_h == info["c_h"]
+        assert np.array_equal(eng.download(i), Uo), (s.ic, s.n)
+
+
+def test_batch_fp64_fixed_steps_and_single_run_agree(hip, oracle):
+    s = mixed_specs(hip, 1, seed=11)[0]
+    U0 = s.state()
+    eng = hip.BatchEngine("fp64", limiter=hip.SlopeLimiter.Minmod, hbm_budget=1 << 28)
+    i = eng.add_steps(U0, 13, gamma=s.gamma, bc=s.bc)
+    eng.advance()
+    r, fin = eng.result(i)
+    assert fin and r.steps == 13
+    Uo, info = oracle_run(oracle, s, U0, MINMOD, steps=13)
+    assert np.array_equal(eng.download(i), Uo)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_same_alone_and_in_the_mix(hip, precision):
+    """Continuous batching must not change a result: instance X alone, then X admitted into a
+    running mix (others arriving before and after it) -> identical bits."""
+    specs = mixed_specs(hip, 7, seed=5)
+    x = specs[3]
+    Ux = x.state()
+    alone = hip.BatchEngine(precision, hbm_budget=1 << 30)
+    ia = alone.add(Ux, x.t_end, gamma=x.gamma, bc=x.bc)
+    alone.advance()
+    ref = alone.download(ia)
+    ref_r, _ = alone.result(ia)
+
+    mix = hip.BatchEngine(precision, hbm_budget=1 << 30)
+    early = [mix.add(s.state(), s.t_end, gamma=s.gamma, bc=s.bc) for s in specs[:3]]
+    mix.advance(max_replays=2)                       # others are already running
+    ix = mix.add(Ux, x.t_end, gamma=x.gamma, bc=x.bc)
+    mix.advance(max_replays=1)
+    late = [mix.add(s.state(), s.t_end, gamma=s.gamma, bc=s.bc) for s in specs[4:]]
+    mix.remove(early[0])                             # a departure mid-run
+    mix.advance()
+    r, fin = mix.result(ix)
+    assert fin and r == ref_r
+    assert np.array_equal(mix.download(ix), ref)
+    for i in early[1:] + late:
+        assert mix.result(i)[1]
+
+
+def test_admission_by_hbm_budget(hip):
+    per = hip.instance_bytes("fp32", 64, 64)
+    eng = hip.BatchEngine("fp32", hbm_budget=3 * per, max_instances=8)
+    U = hip.init_ic("orszag_tang", 64, 64)
+    ids = [eng.add_steps(U, 4) for _ in range(3)]
+    with pytest.raises(hip.CapacityFull):
+        eng.add_steps(U, 4)
+    eng.remove(ids[1])
+    ids.append(eng.add_steps(U, 4))
+    st = eng.stats()
+    assert st["instances"] == 3 and st["bytes_used"] <= st["bytes_budget"]
+    eng.advance()
+    outs = [eng.download(i) for i in (ids[0], ids[2], ids[3])]
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
+@pytest.mark.parametrize("rows", [1, 5, 200])
+def test_batch_tiling_independent(hip, oracle, rows):
+    s = mixed_specs(hip, 1, seed=17)[0]
+    U0 = s.state()
+    eng = hip.BatchEngine("fp64", limiter=hip.SlopeLimiter.MC, hbm_budget=1 << 28)
+    eng.set_seg_rows(rows)
+    i = eng.add(U0, s.t_end, gamma=s.gamma, bc=s.bc)
+    eng.advance()
+    Uo, info = oracle_run(oracle, s, U0, MC)
+    assert eng.result(i)[0].steps == info["steps"]
+    assert np.array_equal(eng.download(i), Uo)
