@@ -1,18 +1,21 @@
 """Benchmark of the hip backend's hot path: SSP-RK2 steps of the GLM-MHD MUSCL/HLL stencil.
 
-Default workload (one GPU): Orszag-Tang vortex at 4096 x 4096 on [0,1]^2, periodic, gamma 5/3,
-B0 = 1/sqrt(4 pi), minmod, HLL, CFL 0.4, glm_alpha 0.1. A "step" is one full SSP-RK2 step of
-the whole grid (two fused stage kernels, CFL reduction fused into stage 2). The state is
-resident in HBM before the timed region starts; upload/download are outside it.
+Headline workload (one GPU): the seeded mixed-size ensemble -- 512 independent Orszag-Tang /
+rotor instances, square grids drawn uniformly from {64, 96, 128, 192, 256}, gamma ~ U[1.4, 5/3],
+field scale ~ U[0.5, 2], minmod, HLL, CFL 0.4 -- advanced by the batched FP32-bulk engine (one
+persistent kernel per RK stage, per-instance dt). A "step" is one SSP-RK2 step of every instance.
+The states are resident in HBM before the timed region; admission/upload are outside it.
 
-value: fp64 Mcell-updates/s = nx * ny * steps / seconds / 1e6 (the reference's definition,
-tools/mhd_backend_compare.cpp:295,527). The FP32-bulk engine on the same workload is reported
-beside it. roofline: the dominant stage kernel's algorithmic bytes per launch over its average
-duration, from HIP events recorded on the solver's stream around each launch in the timed
-region. cpu_baseline: the reference's own CPU solver (Release flags) timed on this host on a
-bounded sample of the same workload.
+value: batched FP32 Mcell-updates/s = sum(N^2) * steps / seconds / 1e6 (the reference's
+definition, tools/mhd_backend_compare.cpp:295,527), timed with HIP events around every stage
+kernel. Beside it: the same ensemble on the plain fp64 single-run kernel one instance at a time
+(the reference's execution model) and the ratio; the per-field relative L1 of the FP32 results
+against those fp64 results; the batched fp64 engine; the single large run (Orszag-Tang 4096^2)
+in fp64 and FP32. roofline: the dominant stage kernel's algorithmic bytes over its average
+duration. cpu_baseline: the reference's own CPU solver (Release flags) on a bounded sample of
+the same ensemble on this host's cores.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--nx N] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--instances 512] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -38,42 +41,37 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_device(m, nx, ny, precision):
+# ---- single large run (Orszag-Tang N^2) ----------------------------------------------------
+
+def make_device(m, nx, ny, precision, bc=(0, 0)):
     from mhd_gpu_solver_amd import _lib
     grid = _lib.Grid(nx, ny, 2, 0.0, 1.0, 0.0, 1.0)
     dev = m.DeviceSolver(grid, _lib.FP64 if precision == "fp64" else _lib.FP32)
-    dev.configure(0, 0, None, True)
+    dev.configure(bc[0], bc[1], None, True)
     return dev
 
 
-def initial_state(m, nx, ny):
-    U = m.init_ic("orszag_tang_b0", nx, ny, gamma=5.0 / 3.0, b0=B0_STD)
-    return U
-
-
-def params(m):
+def params(m, gamma=5.0 / 3.0):
     p = m.SolveParams()
     p.limiter = m.SlopeLimiter.Minmod
     p.cfl = 0.4
-    p.gamma = 5.0 / 3.0
+    p.gamma = gamma
     p.glm_alpha = 0.1
     return p
 
 
-def gpu_leg(m, nx, ny, precision, steps, warmup, U0):
-    """Times `steps` steps after `warmup`; returns (mcups, ms_per_step, kernel stats, graph mcups)."""
-    dev = make_device(m, nx, ny, precision)
+def single_leg(m, n, precision, steps, warmup, U0):
+    """`steps` timed steps of the single-run solver after `warmup`."""
+    dev = make_device(m, n, n, precision)
     dev.upload(U0)
     dev.fill_boundaries(5.0 / 3.0)
     p = params(m)
     if warmup:
         dev.advance_steps(p, warmup)
-    # production path (graph replays), wall clock around a synchronising call
     t0 = time.perf_counter()
-    dev.advance_steps(p, steps)
+    dev.advance_steps(p, steps)                      # production path: graph replays
     graph_s = time.perf_counter() - t0
-    # timed region with an event pair around every stage kernel
-    dev.set_kernel_timing(True)
+    dev.set_kernel_timing(True)                      # events around every stage kernel
     t0 = time.perf_counter()
     dev.advance_steps(p, steps)
     wall = time.perf_counter() - t0
@@ -84,16 +82,17 @@ def gpu_leg(m, nx, ny, precision, steps, warmup, U0):
     dev.download(U)
     if not np.isfinite(U).all():
         raise RuntimeError(f"{precision}: non-finite state after the timed steps")
-    cells = nx * ny
+    cells = n * n
     return dict(mcups=cells * steps / wall / 1e6, ms_per_step=wall * 1e3 / steps,
-                graph_mcups=cells * steps / graph_s / 1e6, kernels=k, wall=wall, seg_rows=rows)
+                graph_mcups=cells * steps / graph_s / 1e6, kernels=k, seg_rows=rows,
+                cells=cells)
 
 
-def measured_traffic(kernel_prefix, nx, ny, elem_bytes):
+def measured_traffic(kernel_prefix, n, elem_bytes):
     """HBM bytes per launch of a stage kernel from the committed FETCH_SIZE/WRITE_SIZE summary
     (profiles/traffic_k_stage_orszag_tang_<n>.json), or None when none exists for this size."""
-    path = ROOT / "profiles" / f"traffic_k_stage_orszag_tang_{nx}.json"
-    if nx != ny or not path.exists():
+    path = ROOT / "profiles" / f"traffic_k_stage_orszag_tang_{n}.json"
+    if not path.exists():
         return None
     data = json.loads(path.read_text())
     want = "double" if elem_bytes == 8 else "float"
@@ -103,34 +102,104 @@ def measured_traffic(kernel_prefix, nx, ny, elem_bytes):
     return None
 
 
-def roofline(leg, nx, ny, elem_bytes):
-    cells = nx * ny
-    # algorithmic bytes per launch: stage 1 reads U, writes U*; stage 2 reads U*, U, writes U
+def roofline(kernels, cells, ms_per_step, elem_bytes, name, traffic=None):
+    """Dominant stage kernel: algorithmic bytes per launch (stage 1 reads U, writes U*: 18
+    values per interior cell; stage 2 reads U*, U, writes U: 27) over its mean duration."""
     per_cell = {0: 18 * elem_bytes, 1: 27 * elem_bytes}
-    avg = [leg["kernels"][i][0] / max(1, leg["kernels"][i][1]) for i in (0, 1)]  # ms
+    avg = [kernels[i][0] / max(1, kernels[i][1]) for i in (0, 1)]  # ms
     dom = 0 if avg[0] > avg[1] else 1
     bytes_launch = per_cell[dom] * cells
     achieved = bytes_launch / (avg[dom] * 1e-3) / 1e9
-    step_gbs = 45 * elem_bytes * cells / (leg["ms_per_step"] * 1e-3) / 1e9
+    step_gbs = 45 * elem_bytes * cells / (ms_per_step * 1e-3) / 1e9
     return {
-        "bound": "hbm",
-        "kernel": f"k_stage<{dom + 1}>",
-        "achieved": round(achieved, 1),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": measured_traffic(f"k_stage<{dom + 1}", nx, ny, elem_bytes),
-        "bytes_per_launch": bytes_launch,
-        "avg_launch_ms": round(avg[dom], 4),
-        "seg_rows": leg["seg_rows"],
+        "bound": "hbm", "kernel": f"{name}<{dom + 1}>", "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic(dom) if traffic else None,
+        "bytes_per_launch": bytes_launch, "avg_launch_ms": round(avg[dom], 4),
         "stage_avg_ms": [round(a, 4) for a in avg],
-        "step_achieved_gbs": round(step_gbs, 1),
-        "step_frac": round(step_gbs / HBM_PEAK_GBS, 4),
+        "step_achieved_gbs": round(step_gbs, 1), "step_frac": round(step_gbs / HBM_PEAK_GBS, 4),
     }
 
 
-def cpu_leg(m, nx, ny, U0, budget_s=12.0):
-    """Reference Release build on this host's cores, bounded sample of the same workload."""
+# ---- ensemble -------------------------------------------------------------------------------
+
+def ensemble_batched(m, specs, states, precision, steps, warmup, seg_rows=0):
+    """The whole ensemble in one batch engine: `warmup` steps (autotune, graph capture), then
+    `steps` steps timed with events around every stage kernel, then the same on the graph path."""
+    budget = sum(m.instance_bytes(precision, s.n, s.n) for s in specs) + (64 << 20)
+    eng = m.BatchEngine(precision, limiter=m.SlopeLimiter.Minmod, hbm_budget=budget,
+                        max_instances=len(specs))
+    if seg_rows:
+        eng.set_seg_rows(seg_rows)
+
+    def admit(k):
+        return [eng.add_steps(U, k, gamma=s.gamma, bc=s.bc) for s, U in zip(specs, states)]
+
+    ids = admit(warmup)
+    eng.advance()
+    for i in ids:
+        eng.remove(i)
+    ids = admit(steps)
+    eng.set_kernel_timing(True)
+    t0 = time.perf_counter()
+    eng.advance()
+    wall = time.perf_counter() - t0
+    kernels = eng.kernel_times()
+    eng.set_kernel_timing(False)
+    stats = eng.stats()
+    results = [eng.download(i) for i in ids]
+    for i in ids:
+        eng.remove(i)
+    ids = admit(steps)
+    t0 = time.perf_counter()
+    eng.advance()
+    graph_s = time.perf_counter() - t0
+    for i in ids:
+        eng.remove(i)
+    cells = sum(s.n * s.n for s in specs)
+    return dict(mcups=cells * steps / wall / 1e6, ms_per_step=wall * 1e3 / steps,
+                graph_mcups=cells * steps / graph_s / 1e6, kernels=kernels, cells=cells,
+                seg_rows=stats["seg_rows"], blocks=stats["blocks"], results=results)
+
+
+def ensemble_plain(m, specs, states, steps):
+    """Each instance alone on the plain fp64 single-run solver (the reference's one-solve-per-
+    DeviceSolver model); only the stepping calls are timed (state resident before each)."""
+    devs = {}
+    total = 0.0
+    results = []
+    p = params(m)
+    for s, U in zip(specs, states):
+        key = (s.n, s.bc)
+        if key not in devs:
+            devs[key] = make_device(m, s.n, s.n, "fp64", s.bc)
+        dev = devs[key]
+        dev.upload(U)
+        p.gamma = s.gamma
+        t0 = time.perf_counter()
+        dev.advance_steps(p, steps)
+        total += time.perf_counter() - t0
+        out = np.zeros_like(U)
+        dev.download(out)
+        results.append(out)
+    cells = sum(s.n * s.n for s in specs)
+    return dict(mcups=cells * steps / total / 1e6, seconds=total, results=results)
+
+
+def rel_l1_by_field(a, b):
+    i = (slice(2, -2), slice(2, -2))
+    out = []
+    for c in range(8):
+        den = np.abs(b[c][i]).sum()
+        out.append(np.abs(a[c][i] - b[c][i]).sum() / den if den > 0 else 0.0)
+    return out
+
+
+# ---- CPU baseline -------------------------------------------------------------------------
+
+def cpu_leg(specs, states, steps, budget_s=12.0):
+    """The reference's CPU solver (Release flags) on this host, instance by instance over a
+    bounded prefix of the ensemble, each for `steps` steps."""
     sys.path.insert(0, str(ROOT / "tests"))
     import ctypes as C
 
@@ -142,24 +211,24 @@ def cpu_leg(m, nx, ny, U0, budget_s=12.0):
     ref = Reference(path)
     threads = min(16, os.cpu_count() or 1)
     ref.lib.ref_set_threads(threads)
-    p = make_params(limiter=MINMOD, cfl=0.4, gamma=5.0 / 3.0, glm_alpha=0.1)
-    U = np.ascontiguousarray(U0).copy()
-
-    def run(k):
-        return ref.lib.ref_time_steps(nx, ny, 2, 0.0, 1.0, 0.0, 1.0, 0, 0, C.byref(p), k, U)
-
-    run(1)  # thread pool start-up, first touch
-    t2 = run(2)
-    n = int(max(2, min(200, budget_s / max(t2 / 2, 1e-6))))
-    secs = run(n)
+    done_cells = 0
+    secs = 0.0
+    count = 0
+    for s, U0 in zip(specs, states):
+        p = make_params(limiter=MINMOD, cfl=0.4, gamma=s.gamma, glm_alpha=0.1)
+        U = np.ascontiguousarray(U0).copy()
+        secs += ref.lib.ref_time_steps(s.n, s.n, 2, 0.0, 1.0, 0.0, 1.0, s.bc[0], s.bc[1],
+                                       C.byref(p), steps, U)
+        done_cells += s.n * s.n * steps
+        count += 1
+        if secs > budget_s:
+            break
     return {
-        "value": round(nx * ny * n / secs / 1e6, 2),
-        "unit": "Mcell-updates/s",
-        "cores": threads,
+        "value": round(done_cells / secs / 1e6, 2), "unit": "Mcell-updates/s", "cores": threads,
         "kind": "reference",
-        "sample": f"orszag_tang {nx}x{ny} fp64 minmod, {n} SSP-RK2 steps in {secs:.1f} s, "
-                  f"reference CPU solver ({path.name}, {ref.lib.ref_simd_backend().decode()} "
-                  f"SIMD, {threads} threads)",
+        "sample": f"first {count} of {len(specs)} ensemble instances, {steps} SSP-RK2 steps each, "
+                  f"fp64, {secs:.1f} s, reference CPU solver ({path.name}, "
+                  f"{ref.lib.ref_simd_backend().decode()} SIMD, {threads} threads)",
     }
 
 
@@ -184,11 +253,16 @@ def copy_probe(nbytes, repeats=3):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--nx", type=int, default=4096)
+    ap.add_argument("--steps", type=int, default=48, help="timed steps (multiple of 8)")
+    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--instances", type=int, default=512)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--single-n", type=int, default=4096)
+    ap.add_argument("--single-steps", type=int, default=48)
+    ap.add_argument("--no-single", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--only", choices=["ensemble", "single"], default=None)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="also run plain device copies of BYTES (counter scale for profiling)")
     args = ap.parse_args()
@@ -198,66 +272,97 @@ def main():
         sys.exit(2)
 
     import mhd_gpu_solver_amd as m
+    from mhd_gpu_solver_amd import workloads
 
     if not m.hip_available():
         raise SystemExit("no gfx950 device visible to the hip backend")
     if args.copy_probe:
         copy_probe(args.copy_probe)
-    nx = ny = args.nx
-    log(f"building initial state {nx}x{ny}")
-    U0 = initial_state(m, nx, ny)
-    log(f"fp64: {args.warmup} warmup + {args.steps} timed steps")
-    f64 = gpu_leg(m, nx, ny, "fp64", args.steps, args.warmup, U0)
-    log(f"fp64: {f64['mcups']:.1f} Mcell-updates/s ({f64['ms_per_step']:.3f} ms/step)")
-    f32 = None
-    if not args.no_fp32:
-        log(f"fp32-bulk: {args.warmup} warmup + {args.steps} timed steps")
-        f32 = gpu_leg(m, nx, ny, "fp32", args.steps, args.warmup, U0)
-        log(f"fp32: {f32['mcups']:.1f} Mcell-updates/s ({f32['ms_per_step']:.3f} ms/step)")
-    cpu = None
-    if not args.no_cpu:
-        log("cpu baseline (reference build, bounded sample)")
-        # the CPU leg starts from the device-filled initial state
-        dev = make_device(m, nx, ny, "fp64")
-        dev.upload(U0)
-        dev.fill_boundaries(5.0 / 3.0)
-        Uf = np.zeros_like(U0)
-        dev.download(Uf)
-        del dev
-        cpu = cpu_leg(m, nx, ny, Uf)
-        if cpu:
-            log(f"cpu: {cpu['value']} Mcell-updates/s ({cpu['sample']})")
-    out = {
-        "metric": METRIC,
-        "value": round(f64["mcups"], 2),
-        "unit": "Mcell-updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(f64["ms_per_step"], 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {
-            "workload": f"orszag_tang_{nx}x{ny}_single_run",
-            "nx": nx, "ny": ny, "ic": "orszag_tang_b0", "b0": "1/sqrt(4 pi)",
-            "limiter": "minmod", "riemann": "hll", "integrator": "ssp_rk2", "cfl": 0.4,
-            "bc": "periodic", "precision": "fp64",
-        },
-        "graph_path_mcups": round(f64["graph_mcups"], 2),
-        "roofline": roofline(f64, nx, ny, 8),
-        "cpu_baseline": cpu,
-    }
-    if f32:
-        out["fp32_bulk"] = {
-            "value": round(f32["mcups"], 2),
-            "ms_per_step": round(f32["ms_per_step"], 4),
-            "graph_path_mcups": round(f32["graph_mcups"], 2),
-            "speedup_vs_fp64": round(f32["mcups"] / f64["mcups"], 3),
-            "roofline": roofline(f32, nx, ny, 4),
-        }
+    out = {"metric": METRIC, "unit": "Mcell-updates/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "data": "synthetic"}
+
+    if args.only != "single":
+        specs = workloads.ensemble(args.instances, args.seed)
+        log(f"ensemble: {len(specs)} instances, {sum(s.n * s.n for s in specs)} cells")
+        states = [s.state() for s in specs]
+        log(f"batched fp32: {args.warmup} warmup + {args.steps} timed steps")
+        b32 = ensemble_batched(m, specs, states, "fp32", args.steps, args.warmup)
+        log(f"batched fp32: {b32['mcups']:.1f} Mcell-updates/s ({b32['ms_per_step']:.3f} ms/step,"
+            f" graph path {b32['graph_mcups']:.1f})")
+        log("batched fp64")
+        b64 = ensemble_batched(m, specs, states, "fp64", args.steps, args.warmup)
+        log(f"batched fp64: {b64['mcups']:.1f} Mcell-updates/s")
+        plain = None
+        l1 = None
+        if not args.no_plain:
+            log("plain fp64, one instance at a time")
+            plain = ensemble_plain(m, specs, states, args.steps)
+            log(f"plain fp64: {plain['mcups']:.1f} Mcell-updates/s")
+            worst = np.zeros(8)
+            for a, b in zip(b32["results"], plain["results"]):
+                worst = np.maximum(worst, rel_l1_by_field(a, b))
+            l1 = {f: float(f"{v:.3e}") for f, v in zip(m.FIELDS[:8], worst)}
+            # the batched fp64 engine reproduces the single-run fp64 kernel bit for bit
+            same = all(np.array_equal(a, b) for a, b in zip(b64["results"], plain["results"]))
+        out.update({
+            "value": round(b32["mcups"], 2),
+            "ms_per_step": round(b32["ms_per_step"], 4),
+            "dtype": "f32",
+            "config": {
+                "workload": f"ensemble_{len(specs)}_mixed_batched_fp32",
+                "instances": len(specs), "sizes": [64, 96, 128, 192, 256],
+                "cells": b32["cells"], "ics": "orszag_tang_b0 (periodic) / rotor_b0 (outflow)",
+                "gamma": "U[1.4,5/3]", "field_scale": "U[0.5,2]", "seed": args.seed,
+                "limiter": "minmod", "riemann": "hll", "integrator": "ssp_rk2", "cfl": 0.4,
+                "precision": "fp32 bulk, fp64 pressure recovery and fast-speed discriminant",
+            },
+            "graph_path_mcups": round(b32["graph_mcups"], 2),
+            "roofline": roofline(b32["kernels"], b32["cells"], b32["ms_per_step"], 4,
+                                 "k_batch_stage"),
+            "seg_rows": b32["seg_rows"], "persistent_blocks": b32["blocks"],
+            "batched_fp64": {"value": round(b64["mcups"], 2),
+                             "ms_per_step": round(b64["ms_per_step"], 4),
+                             "roofline": roofline(b64["kernels"], b64["cells"],
+                                                  b64["ms_per_step"], 8, "k_batch_stage")},
+        })
+        if plain:
+            out["plain_fp64"] = {"value": round(plain["mcups"], 2),
+                                 "seconds": round(plain["seconds"], 3),
+                                 "model": "single-run fp64 solver, one instance after another"}
+            out["batched_fp32_vs_plain_fp64"] = round(b32["mcups"] / plain["mcups"], 2)
+            out["fp32_rel_l1_max_vs_fp64"] = l1
+            out["fp32_l1_within_1e-4"] = bool(max(l1.values()) <= 1e-4)
+            out["batched_fp64_equals_single_run_fp64"] = bool(same)
+        if not args.no_cpu:
+            log("cpu baseline (reference build, bounded ensemble sample)")
+            out["cpu_baseline"] = cpu_leg(specs, states, args.steps)
+            if out["cpu_baseline"]:
+                log(f"cpu: {out['cpu_baseline']['value']} Mcell-updates/s")
+        else:
+            out["cpu_baseline"] = None
+
+    if not args.no_single and args.only != "ensemble":
+        n = args.single_n
+        log(f"single run {n}x{n}")
+        U0 = m.init_ic("orszag_tang_b0", n, n, gamma=5.0 / 3.0, b0=B0_STD)
+        legs = {}
+        for prec, eb in (("fp64", 8), ("fp32", 4)):
+            leg = single_leg(m, n, prec, args.single_steps, args.warmup, U0)
+            log(f"single {prec}: {leg['mcups']:.1f} Mcell-updates/s ({leg['ms_per_step']:.3f} ms/step)")
+            legs[prec] = {
+                "value": round(leg["mcups"], 2), "ms_per_step": round(leg["ms_per_step"], 4),
+                "graph_path_mcups": round(leg["graph_mcups"], 2), "seg_rows": leg["seg_rows"],
+                "roofline": roofline(leg["kernels"], leg["cells"], leg["ms_per_step"], eb,
+                                     "k_stage",
+                                     traffic=lambda dom, eb=eb: measured_traffic(
+                                         f"k_stage<{dom + 1}", n, eb)),
+            }
+        out["single_run"] = {"workload": f"orszag_tang_{n}x{n}_single_run", **legs}
+        if args.only == "single":
+            out.update({"value": legs["fp64"]["value"], "ms_per_step": legs["fp64"]["ms_per_step"],
+                        "dtype": "f64", "config": {"workload": f"orszag_tang_{n}x{n}_single_run"},
+                        "roofline": legs["fp64"]["roofline"], "cpu_baseline": None})
     print(json.dumps(out), flush=True)
 
 

@@ -238,6 +238,11 @@ int mhd_hip_batch_set_seg_rows(mhd_hip_batch* b, int rows);
 
 int mhd_hip_batch_info(const mhd_hip_batch* b, mhd_hip_batch_stats* stats);
 
+/* Measurement, as mhd_hip_set_kernel_timing: while enabled, each replay's steps are launched
+ * eagerly with a HIP event pair around every stage kernel; totals per stage (ms, launches). */
+int mhd_hip_batch_set_kernel_timing(mhd_hip_batch* b, int enable);
+int mhd_hip_batch_kernel_times(const mhd_hip_batch* b, double total_ms[2], long long launches[2]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -93,6 +93,9 @@ SIGNATURES = [
     ("mhd_hip_batch_download", C.c_int, [C.c_void_p, C.c_int, DP9, C.c_size_t]),
     ("mhd_hip_batch_set_seg_rows", C.c_int, [C.c_void_p, C.c_int]),
     ("mhd_hip_batch_info", C.c_int, [C.c_void_p, C.POINTER(BatchStats)]),
+    ("mhd_hip_batch_set_kernel_timing", C.c_int, [C.c_void_p, C.c_int]),
+    ("mhd_hip_batch_kernel_times", C.c_int, [C.c_void_p, C.POINTER(C.c_double),
+                                             C.POINTER(C.c_longlong)]),
 ]
 
 

@@ -96,6 +96,16 @@ class BatchEngine:
     def set_seg_rows(self, rows: int):
         _lib.check(self._l.mhd_hip_batch_set_seg_rows(self.handle, int(rows)))
 
+    def set_kernel_timing(self, enable: bool):
+        _lib.check(self._l.mhd_hip_batch_set_kernel_timing(self.handle, int(enable)))
+
+    def kernel_times(self):
+        """[(total_ms, launches) of the stage-1 kernel, same for stage 2]"""
+        ms = (C.c_double * 2)()
+        n = (C.c_longlong * 2)()
+        _lib.check(self._l.mhd_hip_batch_kernel_times(self.handle, ms, n))
+        return [(ms[0], n[0]), (ms[1], n[1])]
+
     def stats(self) -> dict:
         st = _lib.BatchStats()
         _lib.check(self._l.mhd_hip_batch_info(self.handle, C.byref(st)))

@@ -158,6 +158,13 @@ struct mhd_hip_batch {
     double last_seconds = 0.0;
     hipGraphExec_t graph = nullptr;
 
+    // kernel timing mode: replays are launched eagerly with events around each stage kernel
+    bool kernel_timing = false;
+    double stage_ms[2] = {0.0, 0.0};
+    long long stage_launches[2] = {0, 0};
+    long long timed_cell_steps = 0;  // interior cells x steps of the active instances timed
+    std::vector<hipEvent_t> tev;
+
     ~mhd_hip_batch()
     {
         if (device >= 0) hipSetDevice(device);
@@ -173,9 +180,44 @@ struct mhd_hip_batch {
         if (h_tiles) hipHostFree(h_tiles);
         if (h_ntiles) hipHostFree(h_ntiles);
         if (h_stage) hipHostFree(h_stage);
+        for (hipEvent_t e : tev) hipEventDestroy(e);
         if (ev0) hipEventDestroy(ev0);
         if (ev1) hipEventDestroy(ev1);
         if (stream) hipStreamDestroy(stream);
+    }
+
+    // One replay's worth of steps, launched eagerly with an event pair around each stage
+    // kernel; the kernel times are added after the next synchronisation.
+    template <typename R>
+    void timed_replay()
+    {
+        const size_t need = 4 * kSteps;
+        while (tev.size() < need) {
+            hipEvent_t e;
+            check(hipEventCreate(&e), "hipEventCreate");
+            tev.push_back(e);
+        }
+        for (int k = 0; k < kSteps; ++k) {
+            const int p = k & 1;
+            check(hipEventRecord(tev[4 * k], stream), "hipEventRecord");
+            check(mhdhip::launch_batch_stage<R>(1, limiter, nonideal, tables(), p, blocks, stream),
+                  "batch stage1");
+            check(hipEventRecord(tev[4 * k + 1], stream), "hipEventRecord");
+            check(hipEventRecord(tev[4 * k + 2], stream), "hipEventRecord");
+            check(mhdhip::launch_batch_stage<R>(2, limiter, nonideal, tables(), p, blocks, stream),
+                  "batch stage2");
+            check(hipEventRecord(tev[4 * k + 3], stream), "hipEventRecord");
+        }
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        for (int k = 0; k < kSteps; ++k) {
+            float a = 0.0f, b = 0.0f;
+            check(hipEventElapsedTime(&a, tev[4 * k], tev[4 * k + 1]), "hipEventElapsedTime");
+            check(hipEventElapsedTime(&b, tev[4 * k + 2], tev[4 * k + 3]), "hipEventElapsedTime");
+            stage_ms[0] += a;
+            stage_ms[1] += b;
+        }
+        stage_launches[0] += kSteps;
+        stage_launches[1] += kSteps;
     }
 
     void init(int prec, const mhd_hip_params& tmpl, size_t budget, int max_inst)
@@ -606,8 +648,12 @@ struct mhd_hip_batch {
                 build_tables(rows_for_tiles());
                 dirty = false;
             }
-            ensure_graph<R>();
-            check(hipGraphLaunch(graph, stream), "hipGraphLaunch");
+            if (kernel_timing) {
+                timed_replay<R>();
+            } else {
+                ensure_graph<R>();
+                check(hipGraphLaunch(graph, stream), "hipGraphLaunch");
+            }
         }
         check(hipEventRecord(ev1, stream), "hipEventRecord");
         check(hipEventSynchronize(ev1), "hipEventSynchronize");
@@ -754,6 +800,27 @@ int mhd_hip_batch_set_seg_rows(mhd_hip_batch* b, int rows)
         if (rows < 0) throw InvalidArgument("rows per wave segment must be >= 0");
         b->seg_rows = rows;
         b->dirty = true;
+    });
+}
+
+int mhd_hip_batch_set_kernel_timing(mhd_hip_batch* b, int enable)
+{
+    return guarded([&] {
+        if (!b) throw InvalidArgument("null batch");
+        b->kernel_timing = enable != 0;
+        b->stage_ms[0] = b->stage_ms[1] = 0.0;
+        b->stage_launches[0] = b->stage_launches[1] = 0;
+    });
+}
+
+int mhd_hip_batch_kernel_times(const mhd_hip_batch* b, double* total_ms, long long* launches)
+{
+    return guarded([&] {
+        if (!b || !total_ms || !launches) throw InvalidArgument("null argument");
+        for (int k = 0; k < 2; ++k) {
+            total_ms[k] = b->stage_ms[k];
+            launches[k] = b->stage_launches[k];
+        }
     });
 }
 
