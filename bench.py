@@ -264,7 +264,9 @@ def main():
     ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["ensemble", "single"], default=None)
+    ap.add_argument("--only", choices=["ensemble", "single", "serve"], default=None)
+    ap.add_argument("--serve-requests", type=int, default=128)
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="also run plain device copies of BYTES (counter scale for profiling)")
     args = ap.parse_args()
@@ -365,6 +367,21 @@ def main():
             out.update({"value": legs["fp64"]["value"], "ms_per_step": legs["fp64"]["ms_per_step"],
                         "dtype": "f64", "config": {"workload": f"orszag_tang_{n}x{n}_single_run"},
                         "roofline": legs["fp64"]["roofline"], "cpu_baseline": None})
+    if not args.no_serve and args.only in (None, "serve"):
+        from mhd_gpu_solver_amd.serve import Server
+        reqs = workloads.request_stream(args.serve_requests, rate=200.0, seed=7)
+        log(f"serving: {len(reqs)} requests, Poisson 200/s, N~U{{64..1024}}, t_end~U[0.05,0.5]")
+        srv = Server("fp32", hbm_budget=32 << 30, max_instances=1024)
+        rep = srv.run(reqs, realtime=True)
+        summ = rep.summary()
+        log(f"serving: p50 {summ['p50_ms']} ms, p99 {summ['p99_ms']} ms, "
+            f"{summ['mcups']} Mcell-updates/s, max concurrent {summ['max_concurrent']}")
+        out["serving"] = {"workload": "request_stream_poisson_200", "precision": "fp32",
+                          "limiter": "mc", "hbm_budget_bytes": 32 << 30, **summ}
+        if args.only == "serve":
+            out.update({"value": summ["mcups"], "dtype": "f32", "config": {
+                "workload": "request_stream_poisson_200"}, "roofline": None,
+                "cpu_baseline": None, "ms_per_step": None})
     print(json.dumps(out), flush=True)
 
 

@@ -1,0 +1,125 @@
+"""Continuous-batching request loop over the batched engine.
+
+Solve requests (initial condition, grid, gamma, boundaries, t_end) arrive over time. Arrived
+requests wait in a FIFO queue and are admitted into the running batch as soon as its HBM budget
+and instance slots allow; the engine advances everything admitted by one graph replay at a
+time; finished solves are downloaded, released and their latency (completion - arrival)
+recorded. New requests join between replays while the others keep running, so their work
+overlaps on the device. No counterpart exists in the reference, which solves one grid per
+call (include/mhd/solver_api.hpp:294-321).
+"""
+from __future__ import annotations
+
+import collections
+import dataclasses
+import time
+
+import numpy as np
+
+from .batch import BatchEngine, CapacityFull, instance_bytes
+from .solver import SlopeLimiter
+
+
+@dataclasses.dataclass
+class Completed:
+    index: int
+    n: int
+    ic: str
+    arrival: float
+    admitted: float
+    finished: float
+    steps: int
+    t: float
+    state: np.ndarray | None = None
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+    @property
+    def cell_updates(self):
+        return self.n * self.n * self.steps
+
+
+@dataclasses.dataclass
+class ServeReport:
+    completed: list
+    makespan: float
+    max_concurrent: int
+    replays: int
+
+    def latencies_ms(self):
+        return np.array([c.latency for c in self.completed]) * 1e3
+
+    def summary(self) -> dict:
+        lat = self.latencies_ms()
+        cu = sum(c.cell_updates for c in self.completed)
+        return {
+            "requests": len(self.completed),
+            "p50_ms": round(float(np.percentile(lat, 50)), 2) if len(lat) else None,
+            "p99_ms": round(float(np.percentile(lat, 99)), 2) if len(lat) else None,
+            "mean_ms": round(float(lat.mean()), 2) if len(lat) else None,
+            "makespan_s": round(self.makespan, 3),
+            "mcups": round(cu / self.makespan / 1e6, 2) if self.makespan > 0 else None,
+            "cell_updates": cu,
+            "max_concurrent": self.max_concurrent,
+            "replays": self.replays,
+        }
+
+
+class Server:
+    def __init__(self, precision="fp32", limiter=SlopeLimiter.MC, hbm_budget=32 << 30,
+                 max_instances=1024, cfl=0.4, glm_alpha=0.1):
+        self.engine = BatchEngine(precision, limiter=limiter, hbm_budget=hbm_budget,
+                                  max_instances=max_instances)
+        self.precision = precision
+        self.cfl = cfl
+        self.glm_alpha = glm_alpha
+        self.hbm_budget = hbm_budget
+
+    def run(self, requests, realtime=True, keep_states=False, clock=time.perf_counter):
+        """Serve `requests` (workloads.Spec with .arrival seconds, .t_end) to completion.
+        realtime: arrivals follow the wall clock from the start of the call; otherwise every
+        request is available at once."""
+        order = sorted(range(len(requests)), key=lambda k: requests[k].arrival)
+        pending = collections.deque(order)
+        queue = collections.deque()
+        active = {}
+        done = []
+        t0 = clock()
+        max_conc = 0
+        replays = 0
+        while pending or queue or active:
+            now = clock() - t0
+            while pending and (not realtime or requests[pending[0]].arrival <= now):
+                queue.append(pending.popleft())
+            while queue:
+                k = queue[0]
+                req = requests[k]
+                if instance_bytes(self.precision, req.n, req.n) > self.hbm_budget:
+                    raise ValueError(f"request {k} ({req.n}^2) exceeds the whole HBM budget")
+                try:
+                    iid = self.engine.add(req.state(), req.t_end, cfl=self.cfl, gamma=req.gamma,
+                                          glm_alpha=self.glm_alpha, bc=req.bc)
+                except CapacityFull:
+                    break
+                queue.popleft()
+                active[iid] = (k, clock() - t0)
+            max_conc = max(max_conc, len(active))
+            if not active:
+                if pending and realtime:
+                    time.sleep(max(0.0, requests[pending[0]].arrival - (clock() - t0)))
+                continue
+            self.engine.advance(max_replays=1)
+            replays += 1
+            for iid in list(active):
+                res, fin = self.engine.result(iid)
+                if not fin:
+                    continue
+                k, adm = active.pop(iid)
+                state = self.engine.download(iid) if keep_states else None
+                self.engine.remove(iid)
+                req = requests[k]
+                done.append(Completed(k, req.n, req.ic, req.arrival if realtime else 0.0, adm,
+                                      clock() - t0, res.steps, res.t, state))
+        return ServeReport(done, clock() - t0, max_conc, replays)
