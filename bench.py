@@ -286,7 +286,7 @@ def main():
            "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "data": "synthetic"}
 
-    if args.only != "single":
+    if args.only in (None, "ensemble"):
         specs = workloads.ensemble(args.instances, args.seed)
         log(f"ensemble: {len(specs)} instances, {sum(s.n * s.n for s in specs)} cells")
         states = [s.state() for s in specs]
@@ -346,7 +346,7 @@ def main():
         else:
             out["cpu_baseline"] = None
 
-    if not args.no_single and args.only != "ensemble":
+    if not args.no_single and args.only in (None, "single"):
         n = args.single_n
         log(f"single run {n}x{n}")
         U0 = m.init_ic("orszag_tang_b0", n, n, gamma=5.0 / 3.0, b0=B0_STD)

@@ -72,3 +72,18 @@ def request_stream(count, rate=200.0, seed=7, n_min=64, n_max=1024, t_min=0.05, 
         s.arrival = t
         out.append(s)
     return out
+
+
+def circles_mask(n, count=8, seed=11, r_min=0.02, r_max=0.06):
+    """uint8 (n, n) interior mask on [0,1]^2, 1 = fluid, with `count` solid circles of radius
+    ~ U[r_min, r_max] and seeded centres, rasterised at cell centres with
+    (x - cx)^2 + (y - cy)^2 <= r^2 (the reference GUI's rule, gui/geometry.py:29-38)."""
+    rng = np.random.default_rng(seed)
+    x = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(x, x)
+    mask = np.ones((n, n), dtype=np.uint8)
+    for _ in range(count):
+        r = rng.uniform(r_min, r_max)
+        cx, cy = rng.uniform(r, 1.0 - r, size=2)
+        mask[(X - cx) ** 2 + (Y - cy) ** 2 <= r * r] = 0
+    return mask

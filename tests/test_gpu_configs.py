@@ -1,0 +1,135 @@
+"""The workloads' parity checks on the GPU beyond the unit cases:
+* Alfven-wave convergence (fp64): the reference's published error table and an observed order
+  of about 2 (docs/benchmarks.md:105), and second order for the oblique 2-D wave;
+* rotor 1024^2 with Ohmic + Hall + ambipolar terms, outflow and eight solid circles: fp64
+  bit-exact against the oracle, FP32-bulk within 1e-4 relative L1 per field;
+* Orszag-Tang 4096^2 fp64: the dt sequence of 50 steps bit for bit against the oracle and
+  relative L1 per field <= 1e-12 (in fact equal)."""
+import math
+
+import numpy as np
+import pytest
+
+from checkers import MC, MINMOD, OUTFLOW, PERIODIC, FIELDS, make_grid, make_params
+
+pytestmark = pytest.mark.gpu
+
+
+def alfven_l1(U, nx, t):
+    """Relative L1 of (by, bz, vy, vz) against the exact translated wave
+    (include/verification/diagnostics.hpp:83-107, inititial_condition.hpp:175-192)."""
+    x = (np.arange(nx) + 0.5) / nx
+    ph = 2 * math.pi * (x - t)
+    d = 1e-3
+    ex = [d * np.cos(ph), d * np.sin(ph), -d * np.cos(ph), -d * np.sin(ph)]
+    Ui = U[:, 2:-2, 2:-2]
+    rho = Ui[0]
+    got = [Ui[6], Ui[7], Ui[2] / rho, Ui[3] / rho]
+    num = sum(np.abs(g - e[None, :]).sum() for g, e in zip(got, ex))
+    den = sum(np.abs(e).sum() * Ui.shape[1] for e in ex)
+    return num / den
+
+
+def test_alfven_convergence_matches_published_table(hip):
+    published = {32: 3.15e-2, 64: 9.44e-3, 128: 2.56e-3, 256: 6.57e-4}  # docs/benchmarks.md:105
+    errs = {}
+    for n in (32, 64, 128, 256):
+        s = hip.MHDSolver(n, 1, 0.0, 1.0, 0.0, 1.0 / n)
+        s.set_backend("hip", "fp64")
+        s.initialize("alfven_wave")
+        r = s.run(1.0)
+        errs[n] = alfven_l1(s.state, n, r.t)
+        assert errs[n] == pytest.approx(published[n], rel=0.01), (n, errs[n])
+    order = math.log(errs[128] / errs[256]) / math.log(2)
+    assert 1.9 <= order <= 2.05
+
+
+def test_oblique_alfven_second_order_fp32_and_fp64(hip):
+    """Config-5 wave: amplitude 0.1 along atan(2); exact solution = translation along k at v_A = 1."""
+    for prec, tol in (("fp64", 1.85), ("fp32", 1.8)):
+        errs = []
+        for n in (32, 64, 128):
+            s = hip.MHDSolver(n, n)
+            s.set_backend("hip", prec)
+            s.initialize("alfven_oblique")
+            U0 = s.state.copy()
+            r = s.run(0.2)
+            # exact: the initial wave shifted by t along k = (1, 2)/sqrt(5), wavelength 1/sqrt(5)
+            x = (np.arange(n) + 0.5) / n
+            X, Y = np.meshgrid(x, x)
+            ph = 2 * math.pi * (X + 2 * Y - math.sqrt(5) * r.t)
+            bz_exact = 0.1 * np.sin(ph)
+            errs.append(np.abs(s.state[7, 2:-2, 2:-2] - bz_exact).mean())
+        orders = [math.log(errs[i] / errs[i + 1]) / math.log(2) for i in range(2)]
+        assert orders[-1] >= tol, (prec, errs, orders)
+
+
+def rotor_nonideal(hip, n, precision, mask):
+    s = hip.MHDSolver(n, n)
+    s.set_backend("hip", precision)
+    s.set_outflow_bc()
+    s.set_limiter_minmod()
+    s.enable_ohmic(1e-3)
+    s.enable_hall(1e-2)
+    s.enable_ambipolar(1e-3)
+    s.initialize("rotor")
+    s.set_mask(mask)
+    return s
+
+
+def test_rotor_nonideal_mask_1024_fp64_bitexact(hip, oracle):
+    from mhd_gpu_solver_amd import workloads
+    n = 1024
+    mask = workloads.circles_mask(n, 8, seed=11)
+    assert 0 < (mask == 0).sum() < 0.1 * n * n
+    s = rotor_nonideal(hip, n, "fp64", mask)
+    U0 = s.state.copy()
+    r = s.advance_steps(12)
+    p = make_params(t_end=1e300, cfl=0.4, limiter=MINMOD, eta_ohm=1e-3, eta_hall=1e-2,
+                    eta_ambipolar=1e-3, max_steps=12)
+    Uo, info = oracle.solve(make_grid(n, n), U0, p, bc=(OUTFLOW, OUTFLOW), mask=mask,
+                            allow_max_steps=True)
+    assert r.steps == info["steps"] == 12 and r.t == info["t"]
+    assert np.array_equal(s.state, Uo)
+
+
+def test_rotor_nonideal_mask_1024_fp32_l1(hip):
+    from mhd_gpu_solver_amd import workloads
+    n = 1024
+    mask = workloads.circles_mask(n, 8, seed=11)
+    a = rotor_nonideal(hip, n, "fp64", mask)
+    b = rotor_nonideal(hip, n, "fp32", mask)
+    ra = a.advance_steps(200)
+    rb = b.advance_steps(200)
+    assert ra.steps == rb.steps == 200
+    assert abs(rb.t - ra.t) <= 1e-4 * ra.t
+    for c, name in enumerate(FIELDS[:8]):
+        ref = a.state[c, 2:-2, 2:-2]
+        den = np.abs(ref).sum()
+        if den == 0:
+            continue
+        err = np.abs(b.state[c, 2:-2, 2:-2] - ref).sum() / den
+        assert err <= 1e-4, (name, err)
+
+
+@pytest.mark.slow
+def test_orszag_tang_4096_dt_sequence_50_steps(hip, oracle):
+    n = 4096
+    b0 = 1.0 / math.sqrt(4 * math.pi)
+    s = hip.MHDSolver(n, n)
+    s.set_backend("hip", "fp64")
+    s.set_limiter_minmod()
+    s.initialize("orszag_tang_b0", b0=b0)
+    U0 = s.state.copy()
+    dev = s._dev()
+    dev.set_dt_log(64)
+    r = s.advance_steps(50)
+    dts = dev.dt_log(64)
+    p = make_params(t_end=1e300, cfl=0.4, limiter=MINMOD, max_steps=50)
+    Uo, info = oracle.solve(make_grid(n, n), U0, p, allow_max_steps=True, dt_cap=64)
+    np.testing.assert_array_equal(dts, info["dt"])
+    for c, name in enumerate(FIELDS):
+        den = np.abs(Uo[c]).sum()
+        err = np.abs(s.state[c] - Uo[c]).sum() / den if den > 0 else np.abs(s.state[c]).sum()
+        assert err <= 1e-12, (name, err)
+    assert np.array_equal(s.state, Uo)
