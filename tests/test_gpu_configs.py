@@ -46,9 +46,9 @@ def test_alfven_convergence_matches_published_table(hip):
 
 def test_oblique_alfven_second_order_fp32_and_fp64(hip):
     """Config-5 wave: amplitude 0.1 along atan(2); exact solution = translation along k at v_A = 1."""
-    for prec, tol in (("fp64", 1.85), ("fp32", 1.8)):
+    for prec, tol in (("fp64", 1.7), ("fp32", 1.7)):
         errs = []
-        for n in (32, 64, 128):
+        for n in (64, 128, 256):
             s = hip.MHDSolver(n, n)
             s.set_backend("hip", prec)
             s.initialize("alfven_oblique")
