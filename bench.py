@@ -71,10 +71,14 @@ def single_leg(m, n, precision, steps, warmup, U0):
     t0 = time.perf_counter()
     dev.advance_steps(p, steps)                      # production path: graph replays
     graph_s = time.perf_counter() - t0
+    tot0 = dev.totals()
     dev.set_kernel_timing(True)                      # events around every stage kernel
     t0 = time.perf_counter()
     dev.advance_steps(p, steps)
     wall = time.perf_counter() - t0
+    tot1 = dev.totals()
+    drift = {"mass": float(f"{abs(tot1[0] - tot0[0]) / abs(tot0[0]):.3e}"),
+             "energy": float(f"{abs(tot1[4] - tot0[4]) / abs(tot0[4]):.3e}")}
     k = dev.kernel_times()
     rows = dev.seg_rows()
     dev.set_kernel_timing(False)
@@ -85,7 +89,7 @@ def single_leg(m, n, precision, steps, warmup, U0):
     cells = n * n
     return dict(mcups=cells * steps / wall / 1e6, ms_per_step=wall * 1e3 / steps,
                 graph_mcups=cells * steps / graph_s / 1e6, kernels=k, seg_rows=rows,
-                cells=cells)
+                cells=cells, drift=drift)
 
 
 def measured_traffic(kernel_prefix, n, elem_bytes):
@@ -140,6 +144,7 @@ def ensemble_batched(m, specs, states, precision, steps, warmup, seg_rows=0):
     for i in ids:
         eng.remove(i)
     ids = admit(steps)
+    before = {i: eng.totals(i) for i, s in zip(ids, specs) if s.bc == (0, 0)}
     eng.set_kernel_timing(True)
     t0 = time.perf_counter()
     eng.advance()
@@ -147,6 +152,11 @@ def ensemble_batched(m, specs, states, precision, steps, warmup, seg_rows=0):
     kernels = eng.kernel_times()
     eng.set_kernel_timing(False)
     stats = eng.stats()
+    drift = [0.0, 0.0]
+    for i, t_b in before.items():
+        t_a = eng.totals(i)
+        for k, c in enumerate((0, 4)):  # mass, energy of the periodic (closed) instances
+            drift[k] = max(drift[k], abs(t_a[c] - t_b[c]) / abs(t_b[c]))
     results = [eng.download(i) for i in ids]
     for i in ids:
         eng.remove(i)
@@ -159,7 +169,9 @@ def ensemble_batched(m, specs, states, precision, steps, warmup, seg_rows=0):
     cells = sum(s.n * s.n for s in specs)
     return dict(mcups=cells * steps / wall / 1e6, ms_per_step=wall * 1e3 / steps,
                 graph_mcups=cells * steps / graph_s / 1e6, kernels=kernels, cells=cells,
-                seg_rows=stats["seg_rows"], blocks=stats["blocks"], results=results)
+                seg_rows=stats["seg_rows"], blocks=stats["blocks"], results=results,
+                drift={"mass": float(f"{drift[0]:.3e}"), "energy": float(f"{drift[1]:.3e}"),
+                       "instances": len(before)})
 
 
 def ensemble_plain(m, specs, states, steps):
@@ -325,7 +337,9 @@ def main():
             "roofline": roofline(b32["kernels"], b32["cells"], b32["ms_per_step"], 4,
                                  "k_batch_stage"),
             "seg_rows": b32["seg_rows"], "persistent_blocks": b32["blocks"],
+            "drift_fp32_periodic_max": b32["drift"],
             "batched_fp64": {"value": round(b64["mcups"], 2),
+                             "drift_periodic_max": b64["drift"],
                              "ms_per_step": round(b64["ms_per_step"], 4),
                              "roofline": roofline(b64["kernels"], b64["cells"],
                                                   b64["ms_per_step"], 8, "k_batch_stage")},
@@ -357,6 +371,7 @@ def main():
             legs[prec] = {
                 "value": round(leg["mcups"], 2), "ms_per_step": round(leg["ms_per_step"], 4),
                 "graph_path_mcups": round(leg["graph_mcups"], 2), "seg_rows": leg["seg_rows"],
+                "drift": leg["drift"],
                 "roofline": roofline(leg["kernels"], leg["cells"], leg["ms_per_step"], eb,
                                      "k_stage",
                                      traffic=lambda dom, eb=eb: measured_traffic(

@@ -133,6 +133,12 @@ int mhd_hip_dt_log(mhd_hip_solver* s, double* out, int capacity, int* count);
 #define MHD_HIP_FILL_WALLS 2
 int mhd_hip_fill_boundaries(mhd_hip_solver* s, double gamma, int what);
 
+/* Totals of the nine conserved components over the interior cells of the device state
+ * (multiply by dx*dy for integrals), summed on the device in double-double arithmetic in a
+ * fixed order: ~106-bit sums whatever the storage precision, identical on every call. Used for
+ * mass/energy drift; the reference sums plainly on the host (diagnostics.hpp:42-57). */
+int mhd_hip_totals(mhd_hip_solver* s, double sums[9]);
+
 /* Seconds of device time spent in the last solve/advance call (HIP events on the solver's
  * stream around the step loop, excluding upload/download). */
 int mhd_hip_last_solve_seconds(const mhd_hip_solver* s, double* seconds);
@@ -232,6 +238,9 @@ int mhd_hip_batch_result(mhd_hip_batch* b, int id, mhd_hip_result* result, int* 
 
 /* Ghost fill (the solve's final fill) and copy of the instance's state to nine fp64 planes. */
 int mhd_hip_batch_download(mhd_hip_batch* b, int id, double* const comps[9], size_t n);
+
+/* mhd_hip_totals for one instance (interior, double-double, deterministic). */
+int mhd_hip_batch_totals(mhd_hip_batch* b, int id, double sums[9]);
 
 /* Rows per wave for the batch kernel; 0 (default) autotunes on the first advance. */
 int mhd_hip_batch_set_seg_rows(mhd_hip_batch* b, int rows);

@@ -72,6 +72,7 @@ SIGNATURES = [
                                  C.POINTER(C.c_int)]),
     ("mhd_hip_fill_boundaries", C.c_int, [C.c_void_p, C.c_double, C.c_int]),
     ("mhd_hip_last_solve_seconds", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    ("mhd_hip_totals", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     ("mhd_hip_set_seg_rows", C.c_int, [C.c_void_p, C.c_int]),
     ("mhd_hip_get_seg_rows", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     ("mhd_hip_set_kernel_timing", C.c_int, [C.c_void_p, C.c_int]),
@@ -91,6 +92,7 @@ SIGNATURES = [
     ("mhd_hip_batch_result", C.c_int, [C.c_void_p, C.c_int, C.POINTER(Result),
                                        C.POINTER(C.c_int)]),
     ("mhd_hip_batch_download", C.c_int, [C.c_void_p, C.c_int, DP9, C.c_size_t]),
+    ("mhd_hip_batch_totals", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     ("mhd_hip_batch_set_seg_rows", C.c_int, [C.c_void_p, C.c_int]),
     ("mhd_hip_batch_info", C.c_int, [C.c_void_p, C.POINTER(BatchStats)]),
     ("mhd_hip_batch_set_kernel_timing", C.c_int, [C.c_void_p, C.c_int]),

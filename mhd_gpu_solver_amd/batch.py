@@ -93,6 +93,12 @@ class BatchEngine:
         _lib.check(self._l.mhd_hip_batch_download(self.handle, int(iid), ptrs, n))
         return U
 
+    def totals(self, iid: int):
+        """Compensated device sums of the nine conserved components of one instance."""
+        out = (C.c_double * 9)()
+        _lib.check(self._l.mhd_hip_batch_totals(self.handle, int(iid), out))
+        return np.array(out[:])
+
     def set_seg_rows(self, rows: int):
         _lib.check(self._l.mhd_hip_batch_set_seg_rows(self.handle, int(rows)))
 

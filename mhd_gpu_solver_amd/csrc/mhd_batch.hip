@@ -150,6 +150,9 @@ struct mhd_hip_batch {
     double* h_stage = nullptr;  // pinned staging for state upload/download
     size_t h_stage_bytes = 0;
 
+    mhdhip::DD* totals_partial = nullptr;
+    double* totals_out = nullptr;
+
     bool dirty = true;
     int seg_rows = 0;  // 0: autotune on the first advance
     int tuned_rows = 0;
@@ -180,6 +183,8 @@ struct mhd_hip_batch {
         if (h_tiles) hipHostFree(h_tiles);
         if (h_ntiles) hipHostFree(h_ntiles);
         if (h_stage) hipHostFree(h_stage);
+        if (totals_partial) hipFree(totals_partial);
+        if (totals_out) hipFree(totals_out);
         for (hipEvent_t e : tev) hipEventDestroy(e);
         if (ev0) hipEventDestroy(ev0);
         if (ev1) hipEventDestroy(ev1);
@@ -695,6 +700,33 @@ struct mhd_hip_batch {
         }
     }
 
+    void totals(int id, double out[9])
+    {
+        const Slot& s = slot_of(id);
+        hipSetDevice(device);
+        if (!totals_partial) {
+            check(hipMalloc(reinterpret_cast<void**>(&totals_partial),
+                            sizeof(mhdhip::DD) * 9 * mhdhip::kTotalsBlocks),
+                  "hipMalloc(totals)");
+            check(hipMalloc(reinterpret_cast<void**>(&totals_out), 9 * sizeof(double)),
+                  "hipMalloc(totals)");
+        }
+        const Geom g = geom_of(s.desc);
+        void* u = static_cast<char*>(arena_base) + s.off;
+        if (precision == MHD_HIP_FP64) {
+            check(mhdhip::launch_totals<double>(static_cast<double*>(u), g, totals_partial,
+                                                totals_out, stream),
+                  "totals");
+        } else {
+            check(mhdhip::launch_totals<float>(static_cast<float*>(u), g, totals_partial,
+                                               totals_out, stream),
+                  "totals");
+        }
+        check(hipMemcpyAsync(out, totals_out, 9 * sizeof(double), hipMemcpyDeviceToHost, stream),
+              "hipMemcpyAsync(totals)");
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    }
+
     void download(int id, double* const comps[9], size_t n)
     {
         const Slot& s = slot_of(id);
@@ -790,6 +822,14 @@ int mhd_hip_batch_download(mhd_hip_batch* b, int id, double* const comps[9], siz
             if (!comps[c]) throw InvalidArgument("null component pointer");
         }
         b->download(id, comps, n);
+    });
+}
+
+int mhd_hip_batch_totals(mhd_hip_batch* b, int id, double sums[9])
+{
+    return guarded([&] {
+        if (!b || !sums) throw InvalidArgument("null argument");
+        b->totals(id, sums);
     });
 }
 

@@ -296,6 +296,90 @@ __global__ __launch_bounds__(256) void k_wall_fill(R* __restrict__ U, Geom g,
     }
 }
 
+// ---- compensated totals of the conserved fields -----------------------------------------
+// Sums of each component over the interior cells in double-double arithmetic (Knuth's
+// TwoSum), so the totals used for mass/energy drift carry ~106 bits whatever the storage
+// precision. Pass 1 writes one (hi, lo) pair per block and component; pass 2 reduces the
+// pairs in block order. No atomics: the result is deterministic.
+struct DD {
+    double hi, lo;
+};
+
+__device__ __forceinline__ DD dd_add(DD a, double b)
+{
+    const double s = a.hi + b;
+    const double bb = s - a.hi;
+    const double err = (a.hi - (s - bb)) + (b - bb);
+    const double lo = a.lo + err;
+    const double hi = s + lo;
+    return DD{hi, lo - (hi - s)};
+}
+
+__device__ __forceinline__ DD dd_add(DD a, DD b)
+{
+    DD r = dd_add(a, b.hi);
+    return dd_add(r, b.lo);
+}
+
+constexpr int kTotalsBlock = 256;
+
+template <typename R>
+__global__ __launch_bounds__(kTotalsBlock) void k_totals_partial(const R* __restrict__ U, Geom g,
+                                                                 DD* __restrict__ partial)
+{
+    __shared__ DD red[kNC][kTotalsBlock / kWave];
+    const long long ncell = static_cast<long long>(g.nx) * g.ny;
+    DD acc[kNC];
+#pragma unroll
+    for (int c = 0; c < kNC; ++c) {
+        acc[c] = DD{0.0, 0.0};
+    }
+    for (long long k = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; k < ncell;
+         k += static_cast<long long>(gridDim.x) * blockDim.x) {
+        const int jj = static_cast<int>(k / g.nx);
+        const int ii = static_cast<int>(k - static_cast<long long>(jj) * g.nx);
+        const long long cell = static_cast<long long>(jj + g.ng) * g.nxt + (ii + g.ng);
+#pragma unroll
+        for (int c = 0; c < kNC; ++c) {
+            acc[c] = dd_add(acc[c], static_cast<double>(U[c * g.plane + cell]));
+        }
+    }
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < kNC; ++c) {
+        DD v = acc[c];
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const DD o{__shfl_down(v.hi, off), __shfl_down(v.lo, off)};
+            v = dd_add(v, o);
+        }
+        if (lane == 0) {
+            red[c][wave] = v;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < kNC) {
+        DD v = red[threadIdx.x][0];
+        for (int w = 1; w < kTotalsBlock / kWave; ++w) {
+            v = dd_add(v, red[threadIdx.x][w]);
+        }
+        partial[blockIdx.x * kNC + threadIdx.x] = v;
+    }
+}
+
+template <typename R>
+__global__ void k_totals_final(const DD* __restrict__ partial, int blocks, double* __restrict__ out)
+{
+    if (threadIdx.x < kNC) {
+        DD v{0.0, 0.0};
+        for (int b = 0; b < blocks; ++b) {
+            v = dd_add(v, partial[b * kNC + threadIdx.x]);
+        }
+        out[threadIdx.x] = v.hi + v.lo;
+    }
+}
+
 // ---- fused RK stage ------------------------------------------------------------------------
 //
 // One wave owns a strip of 60 interior columns and marches rows [r0, r1). Lane l holds global

@@ -31,6 +31,12 @@ hipError_t launch_stage(int stage, int limiter, bool nonideal, bool remap, const
                         const Geom& g, const Physics& ph, const Tiling& tl, int parity,
                         hipStream_t s);
 
+// Compensated totals of the nine conserved fields over the interior: `partial` must hold
+// kTotalsBlocks * 9 DD values, `out` 9 doubles.
+constexpr int kTotalsBlocks = 512;
+template <typename R>
+hipError_t launch_totals(const R* U, const Geom& g, DD* partial, double* out, hipStream_t s);
+
 // Batched engine: one persistent launch of `blocks` workgroups per stage.
 template <typename R>
 hipError_t launch_batch_stage(int stage, int limiter, bool nonideal, const BatchTables& tb,

@@ -149,6 +149,15 @@ int batch_blocks_per_cu(int limiter, bool nonideal)
     return (e == hipSuccess && n > 0) ? n : 1;
 }
 
+template <typename R>
+hipError_t launch_totals(const R* U, const Geom& g, DD* partial, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_totals_partial<R>), dim3(kTotalsBlocks), dim3(kTotalsBlock), 0, s, U, g,
+                       partial);
+    hipLaunchKernelGGL((k_totals_final<R>), dim3(1), dim3(64), 0, s, partial, kTotalsBlocks, out);
+    return hipGetLastError();
+}
+
 #define MHDHIP_INSTANTIATE(R)                                                                    \
     template hipError_t launch_ctrl_init<R>(Ctrl*, hipStream_t);                                 \
     template hipError_t launch_cfl<R>(const R*, const Geom&, const Physics&, const uint8_t*,     \
@@ -160,6 +169,7 @@ int batch_blocks_per_cu(int limiter, bool nonideal)
                                         const Physics&, const Tiling&, int, hipStream_t);         \
     template hipError_t launch_batch_stage<R>(int, int, bool, const BatchTables&, int, int,       \
                                               hipStream_t);                                       \
-    template int batch_blocks_per_cu<R>(int, bool);
+    template int batch_blocks_per_cu<R>(int, bool);                                              \
+    template hipError_t launch_totals<R>(const R*, const Geom&, DD*, double*, hipStream_t);
 
 }  // namespace mhdhip

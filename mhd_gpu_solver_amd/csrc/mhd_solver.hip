@@ -114,6 +114,9 @@ struct mhd_hip_solver {
     hipGraphExec_t graph_exec = nullptr;
     std::vector<unsigned char> graph_key;
 
+    mhdhip::DD* totals_partial = nullptr;  // compensated-sum scratch, allocated on first use
+    double* totals_out = nullptr;
+
     ~mhd_hip_solver() { release(); }
 
     void release()
@@ -129,6 +132,10 @@ struct mhd_hip_solver {
         if (dt_log) hipFree(dt_log);
         if (mask_dev) hipFree(mask_dev);
         if (plan_dev) hipFree(plan_dev);
+        if (totals_partial) hipFree(totals_partial);
+        if (totals_out) hipFree(totals_out);
+        totals_partial = nullptr;
+        totals_out = nullptr;
         for (hipEvent_t e : timing_events) hipEventDestroy(e);
         timing_events.clear();
         if (ev_start) hipEventDestroy(ev_start);
@@ -692,6 +699,30 @@ struct mhd_hip_solver {
         check(hipStreamSynchronize(stream), "hipStreamSynchronize");
     }
 
+    void totals(double out[9])
+    {
+        hipSetDevice(device);
+        if (!totals_partial) {
+            check(hipMalloc(reinterpret_cast<void**>(&totals_partial),
+                            sizeof(mhdhip::DD) * 9 * mhdhip::kTotalsBlocks),
+                  "hipMalloc(totals)");
+            check(hipMalloc(reinterpret_cast<void**>(&totals_out), 9 * sizeof(double)),
+                  "hipMalloc(totals)");
+        }
+        if (precision == MHD_HIP_FP64) {
+            check(mhdhip::launch_totals<double>(static_cast<double*>(U), geom, totals_partial,
+                                                totals_out, stream),
+                  "totals");
+        } else {
+            check(mhdhip::launch_totals<float>(static_cast<float*>(U), geom, totals_partial,
+                                               totals_out, stream),
+                  "totals");
+        }
+        check(hipMemcpyAsync(out, totals_out, 9 * sizeof(double), hipMemcpyDeviceToHost, stream),
+              "hipMemcpyAsync(totals)");
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    }
+
     void set_dt_log(int cap)
     {
         if (cap < 0) {
@@ -876,6 +907,14 @@ int mhd_hip_fill_boundaries(mhd_hip_solver* s, double gamma, int what)
     return guarded([&] {
         if (!s) throw InvalidArgument("null solver");
         s->fill_boundaries(gamma, what);
+    });
+}
+
+int mhd_hip_totals(mhd_hip_solver* s, double sums[9])
+{
+    return guarded([&] {
+        if (!s || !sums) throw InvalidArgument("null argument");
+        s->totals(sums);
     });
 }
 

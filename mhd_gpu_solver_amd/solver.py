@@ -210,6 +210,12 @@ class DeviceSolver:
         what = (_lib.FILL_GHOSTS if ghosts else 0) | (_lib.FILL_WALLS if walls else 0)
         _lib.check(self._l.mhd_hip_fill_boundaries(self.handle, gamma, what))
 
+    def totals(self):
+        """Compensated device sums of the nine conserved components over the interior."""
+        out = (C.c_double * 9)()
+        _lib.check(self._l.mhd_hip_totals(self.handle, out))
+        return np.array(out[:])
+
     def set_seg_rows(self, rows: int):
         """Rows marched per wavefront (0 = autotuned); results do not depend on it."""
         _lib.check(self._l.mhd_hip_set_seg_rows(self.handle, int(rows)))
@@ -503,6 +509,14 @@ class MHDSolver:
     time = property(lambda self: self._t)
     steps = property(lambda self: self._steps)
     c_h = property(lambda self: self._c_h)
+
+    def conserved_totals(self):
+        """Integrals of the nine conserved components over the domain (sum * dx * dy), summed
+        on the device in double-double arithmetic from the current host state."""
+        dev = self._dev()
+        dev.configure(self._bc[0], self._bc[1], self._active_mask(), self._walls)
+        dev.upload(self._U)
+        return dev.totals() * (self.dx * self.dy)
 
     def max_div_b(self):
         """include/initial_condition/inititial_condition.hpp:273-287"""

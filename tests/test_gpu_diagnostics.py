@@ -1,0 +1,51 @@
+"""On-device compensated totals (mass/energy drift) and drift bounds of the reference's GPU
+gate (docs/gpu.md:129-137: drift <= 1e-11 fp64, <= 2e-6 fp32)."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def fsum_totals(U):
+    return np.array([math.fsum(U[c, 2:-2, 2:-2].ravel().tolist()) for c in range(9)])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_totals_are_exactly_rounded_sums(hip, precision):
+    s = hip.MHDSolver(173, 91)
+    s.set_backend("hip", precision)
+    s.initialize("rotor")
+    s.run(0.01)
+    dev = s._dev()
+    dev.upload(s.state)
+    got = dev.totals()
+    want = fsum_totals(s.state)          # s.state holds the device values exactly
+    np.testing.assert_allclose(got, want, rtol=2.3e-16, atol=1e-300)
+    assert np.array_equal(dev.totals(), got)   # deterministic
+
+
+@pytest.mark.parametrize("precision,bound", [("fp64", 1e-11), ("fp32", 2e-6)])
+def test_mass_energy_drift_periodic(hip, precision, bound):
+    s = hip.MHDSolver(128, 128)
+    s.set_backend("hip", precision)
+    s.set_limiter_mc()
+    s.initialize("orszag_tang")
+    t0 = s.conserved_totals()
+    s.run(0.1)
+    t1 = s.conserved_totals()
+    for c, name in ((0, "mass"), (4, "energy")):
+        drift = abs(t1[c] - t0[c]) / abs(t0[c])
+        assert drift <= bound, (name, drift)
+
+
+def test_batch_totals_match_single_run(hip):
+    from mhd_gpu_solver_amd import workloads
+    spec = workloads.ensemble(3, seed=1)[0]
+    U0 = spec.state()
+    eng = hip.BatchEngine("fp64", limiter=hip.SlopeLimiter.Minmod, hbm_budget=1 << 28)
+    i = eng.add_steps(U0, 16, gamma=spec.gamma, bc=spec.bc)
+    eng.advance()
+    np.testing.assert_allclose(eng.totals(i), fsum_totals(eng.download(i)), rtol=2.3e-16,
+                               atol=1e-300)
