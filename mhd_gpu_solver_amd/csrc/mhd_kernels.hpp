@@ -368,15 +368,24 @@ __global__ __launch_bounds__(kTotalsBlock) void k_totals_partial(const R* __rest
     }
 }
 
+// One wave per component: lane l adds partials l, l+64, ... in order, then a fixed shuffle tree.
 template <typename R>
-__global__ void k_totals_final(const DD* __restrict__ partial, int blocks, double* __restrict__ out)
+__global__ __launch_bounds__(kNC * kWave) void k_totals_final(const DD* __restrict__ partial,
+                                                               int blocks, double* __restrict__ out)
 {
-    if (threadIdx.x < kNC) {
-        DD v{0.0, 0.0};
-        for (int b = 0; b < blocks; ++b) {
-            v = dd_add(v, partial[b * kNC + threadIdx.x]);
-        }
-        out[threadIdx.x] = v.hi + v.lo;
+    const int c = threadIdx.x >> 6;
+    const int lane = threadIdx.x & (kWave - 1);
+    DD v{0.0, 0.0};
+    for (int b = lane; b < blocks; b += kWave) {
+        v = dd_add(v, partial[b * kNC + c]);
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const DD o{__shfl_down(v.hi, off), __shfl_down(v.lo, off)};
+        v = dd_add(v, o);
+    }
+    if (lane == 0) {
+        out[c] = v.hi + v.lo;
     }
 }
 

@@ -154,7 +154,8 @@ hipError_t launch_totals(const R* U, const Geom& g, DD* partial, double* out, hi
 {
     hipLaunchKernelGGL((k_totals_partial<R>), dim3(kTotalsBlocks), dim3(kTotalsBlock), 0, s, U, g,
                        partial);
-    hipLaunchKernelGGL((k_totals_final<R>), dim3(1), dim3(64), 0, s, partial, kTotalsBlocks, out);
+    hipLaunchKernelGGL((k_totals_final<R>), dim3(1), dim3(kNC * kWave), 0, s, partial,
+                       kTotalsBlocks, out);
     return hipGetLastError();
 }
 

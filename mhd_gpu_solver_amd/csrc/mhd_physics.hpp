@@ -52,8 +52,13 @@ __device__ __forceinline__ R vmin(R a, R b) { return (b < a) ? b : a; }
 template <typename R>
 __device__ __forceinline__ R vmax(R a, R b) { return (a < b) ? b : a; }
 
+// fp64: correctly rounded sqrt and division (parity with the CPU path).
+// fp32-bulk: the hardware square root and reciprocal (v_sqrt_f32, v_rcp_f32, ~1 ulp); the FP32
+// path is held to a relative L1 tolerance against fp64, not to bit parity.
 __device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
-__device__ __forceinline__ float dsqrt(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ float dsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double drcp(double x) { return 1.0 / x; }
+__device__ __forceinline__ float drcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double dabs(double x) { return __builtin_fabs(x); }
 __device__ __forceinline__ float dabs(float x) { return __builtin_fabsf(x); }
 
@@ -128,7 +133,7 @@ __device__ __forceinline__ R fast_speed(const R (&W)[kNC], R gamma)
 {
     using D = FaceDir<DIR>;
     const R half(0.5);
-    const R inv_rho = R(1) / W[RHO];
+    const R inv_rho = drcp(W[RHO]);
     const R cs2 = gamma * W[PR] * inv_rho;
     const R bn2 = W[D::BN] * W[D::BN];
     const R b2 = bn2 + W[D::BT] * W[D::BT] + W[BZ] * W[BZ];
@@ -209,7 +214,7 @@ __device__ __forceinline__ void hll_face(R (&WL)[kNC], R (&WR)[kNC], const R (&W
     cons_flux<DIR>(WR, inv_gm1, ch2, UR, FR);
 
     const R denom = sR - sL;
-    const R inv_denom = R(1) / denom;
+    const R inv_denom = drcp(denom);
     const R sLsR = sL * sR;
     const R half(0.5);
     const bool use_left = sL >= zero;
@@ -227,7 +232,7 @@ __device__ __forceinline__ void hll_face(R (&WL)[kNC], R (&WR)[kNC], const R (&W
 template <typename R>
 __device__ __forceinline__ void to_prim(const R (&U)[kNC], R gm1, R (&W)[kNC])
 {
-    const R inv_rho = R(1) / U[RHO];
+    const R inv_rho = drcp(U[RHO]);
     W[RHO] = U[RHO];
     W[VX] = U[1] * inv_rho;
     W[VY] = U[2] * inv_rho;
@@ -243,7 +248,7 @@ __device__ __forceinline__ void to_prim(const R (&U)[kNC], R gm1, R (&W)[kNC])
 template <typename R>
 __device__ __forceinline__ R cell_max_speed(const R (&U)[kNC], R gamma, R gm1)
 {
-    const R inv_rho = R(1) / U[RHO];
+    const R inv_rho = drcp(U[RHO]);
     R W[kNC];
     W[RHO] = U[RHO];
     W[VX] = U[1] * inv_rho;
@@ -291,7 +296,7 @@ __device__ __forceinline__ void nonideal_e(const R (&J)[3], const R (&B)[3], con
     const R jb2 = J[0] * B[1] - J[1] * B[0];
     if (cfg.hall) {
         const R eta = static_cast<R>(cfg.eta_hall);
-        const R invB = R(1) / dsqrt(B2);
+        const R invB = drcp(dsqrt(B2));
         E[0] += eta * jb0 * invB;
         E[1] += eta * jb1 * invB;
         E[2] += eta * jb2 * invB;
@@ -301,7 +306,7 @@ __device__ __forceinline__ void nonideal_e(const R (&J)[3], const R (&B)[3], con
         const R a0 = jb1 * B[2] - jb2 * B[1];
         const R a1 = jb2 * B[0] - jb0 * B[2];
         const R a2 = jb0 * B[1] - jb1 * B[0];
-        const R invB2 = R(1) / B2;
+        const R invB2 = drcp(B2);
         E[0] += eta * a0 * invB2;
         E[1] += eta * a1 * invB2;
         E[2] += eta * a2 * invB2;

@@ -2,7 +2,7 @@
 
     python -m mhd_gpu_solver_amd.profiling trace  <kernel_trace.csv>
     python -m mhd_gpu_solver_amd.profiling traffic <fetch counter_collection.csv> \
-        <write counter_collection.csv> <copy bytes> <nx> <ny>
+        <write counter_collection.csv> <copy bytes> <interior cells per launch>
 
 ``trace`` prints per-kernel launch statistics. Launches shorter than ``NOOP_US`` are the
 early-exit steps at the end of a fixed-size batch and are counted apart from the active ones.
@@ -53,7 +53,7 @@ def _counter(path: str, counter: str):
     return per
 
 
-def traffic(fetch_csv: str, write_csv: str, copy_bytes: int, nx: int, ny: int) -> dict:
+def traffic(fetch_csv: str, write_csv: str, copy_bytes: int, cells: int) -> dict:
     fetch = _counter(fetch_csv, "FETCH_SIZE")
     write = _counter(write_csv, "WRITE_SIZE")
     kib = copy_bytes / 1024.0
@@ -61,11 +61,10 @@ def traffic(fetch_csv: str, write_csv: str, copy_bytes: int, nx: int, ny: int) -
     copy_w = [v for v, d in write["__amd_rocclr_copyBuffer"] if v > 0.1 * kib]
     f_scale = kib / (sum(copy_f) / len(copy_f))
     w_scale = kib / (sum(copy_w) / len(copy_w))
-    cells = nx * ny
     res = {"copy_bytes": copy_bytes, "fetch_scale": round(f_scale, 4),
            "write_scale": round(w_scale, 4), "kernels": {}}
     for k in sorted(set(fetch) | set(write)):
-        if "k_stage" not in k:
+        if "stage" not in k:
             continue
         f = [v for v, d in fetch.get(k, []) if d > NOOP_US]
         w = [v for v, d in write.get(k, []) if d > NOOP_US]
@@ -74,7 +73,7 @@ def traffic(fetch_csv: str, write_csv: str, copy_bytes: int, nx: int, ny: int) -
         fb = sum(f) / len(f) * f_scale * 1024
         wb = sum(w) / len(w) * w_scale * 1024
         elem = 8 if "double" in k else 4
-        stage = 1 if k.startswith("k_stage<1") else 2
+        stage = 1 if "stage<1" in k else 2
         alg_r = (9 if stage == 1 else 18) * elem * cells
         alg_w = 9 * elem * cells
         res["kernels"][k] = {
@@ -89,9 +88,8 @@ def traffic(fetch_csv: str, write_csv: str, copy_bytes: int, nx: int, ny: int) -
 def main(argv):
     if len(argv) >= 2 and argv[0] == "trace":
         print(kernel_table(argv[1]))
-    elif len(argv) >= 6 and argv[0] == "traffic":
-        print(json.dumps(traffic(argv[1], argv[2], int(argv[3]), int(argv[4]), int(argv[5])),
-                         indent=1))
+    elif len(argv) >= 5 and argv[0] == "traffic":
+        print(json.dumps(traffic(argv[1], argv[2], int(argv[3]), int(argv[4])), indent=1))
     else:
         print(__doc__)
         return 2
