@@ -92,10 +92,10 @@ def single_leg(m, n, precision, steps, warmup, U0):
                 cells=cells, drift=drift)
 
 
-def measured_traffic(kernel_prefix, n, elem_bytes):
+def measured_traffic(kernel_prefix, elem_bytes, workload, kernel="k_stage"):
     """HBM bytes per launch of a stage kernel from the committed FETCH_SIZE/WRITE_SIZE summary
-    (profiles/traffic_k_stage_orszag_tang_<n>.json), or None when none exists for this size."""
-    path = ROOT / "profiles" / f"traffic_k_stage_orszag_tang_{n}.json"
+    (profiles/traffic_<kernel>_<workload>.json), or None when none exists for this workload."""
+    path = ROOT / "profiles" / f"traffic_{kernel}_{workload}.json"
     if not path.exists():
         return None
     data = json.loads(path.read_text())
@@ -335,7 +335,10 @@ def main():
             },
             "graph_path_mcups": round(b32["graph_mcups"], 2),
             "roofline": roofline(b32["kernels"], b32["cells"], b32["ms_per_step"], 4,
-                                 "k_batch_stage"),
+                                 "k_batch_stage",
+                                 traffic=lambda dom: measured_traffic(
+                                     f"k_batch_stage<{dom + 1}", 4, f"ensemble_{len(specs)}",
+                                     "k_batch_stage")),
             "seg_rows": b32["seg_rows"], "persistent_blocks": b32["blocks"],
             "drift_fp32_periodic_max": b32["drift"],
             "batched_fp64": {"value": round(b64["mcups"], 2),
@@ -375,7 +378,7 @@ def main():
                 "roofline": roofline(leg["kernels"], leg["cells"], leg["ms_per_step"], eb,
                                      "k_stage",
                                      traffic=lambda dom, eb=eb: measured_traffic(
-                                         f"k_stage<{dom + 1}", n, eb)),
+                                         f"k_stage<{dom + 1}", eb, f"orszag_tang_{n}")),
             }
         out["single_run"] = {"workload": f"orszag_tang_{n}x{n}_single_run", **legs}
         if args.only == "single":

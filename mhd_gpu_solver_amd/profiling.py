@@ -2,7 +2,7 @@
 
     python -m mhd_gpu_solver_amd.profiling trace  <kernel_trace.csv>
     python -m mhd_gpu_solver_amd.profiling traffic <fetch counter_collection.csv> \
-        <write counter_collection.csv> <copy bytes> <interior cells per launch>
+        <write counter_collection.csv> <copy bytes> <interior cells per launch> [kernel filter]
 
 ``trace`` prints per-kernel launch statistics. Launches shorter than ``NOOP_US`` are the
 early-exit steps at the end of a fixed-size batch and are counted apart from the active ones.
@@ -53,7 +53,8 @@ def _counter(path: str, counter: str):
     return per
 
 
-def traffic(fetch_csv: str, write_csv: str, copy_bytes: int, cells: int) -> dict:
+def traffic(fetch_csv: str, write_csv: str, copy_bytes: int, cells: int,
+            name_filter: str = "stage") -> dict:
     fetch = _counter(fetch_csv, "FETCH_SIZE")
     write = _counter(write_csv, "WRITE_SIZE")
     kib = copy_bytes / 1024.0
@@ -64,7 +65,7 @@ def traffic(fetch_csv: str, write_csv: str, copy_bytes: int, cells: int) -> dict
     res = {"copy_bytes": copy_bytes, "fetch_scale": round(f_scale, 4),
            "write_scale": round(w_scale, 4), "kernels": {}}
     for k in sorted(set(fetch) | set(write)):
-        if "stage" not in k:
+        if name_filter not in k:
             continue
         f = [v for v, d in fetch.get(k, []) if d > NOOP_US]
         w = [v for v, d in write.get(k, []) if d > NOOP_US]
@@ -89,7 +90,8 @@ def main(argv):
     if len(argv) >= 2 and argv[0] == "trace":
         print(kernel_table(argv[1]))
     elif len(argv) >= 5 and argv[0] == "traffic":
-        print(json.dumps(traffic(argv[1], argv[2], int(argv[3]), int(argv[4])), indent=1))
+        filt = argv[5] if len(argv) >= 6 else "stage"
+        print(json.dumps(traffic(argv[1], argv[2], int(argv[3]), int(argv[4]), filt), indent=1))
     else:
         print(__doc__)
         return 2
