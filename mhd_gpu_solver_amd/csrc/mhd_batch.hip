@@ -582,7 +582,7 @@ struct mhd_hip_batch {
     {
         int best = 32;
         float best_ms = 1e30f;
-        for (int rows : {8, 16, 32, 64, 128}) {
+        for (int rows : {8, 12, 16, 20, 24, 28, 32, 40, 48, 64, 96, 128}) {
             build_tables(rows);
             check(mhdhip::launch_batch_stage<R>(1, limiter, nonideal, tables(), 0, blocks, stream),
                   "tune");

@@ -37,6 +37,10 @@ constexpr int kTotalsBlocks = 512;
 template <typename R>
 hipError_t launch_totals(const R* U, const Geom& g, DD* partial, double* out, hipStream_t s);
 
+// Workgroups of k_stage (stage 2, the larger) that fit on one CU at once (occupancy query).
+template <typename R>
+int stage_blocks_per_cu(int limiter, bool nonideal, bool remap);
+
 // Batched engine: one persistent launch of `blocks` workgroups per stage.
 template <typename R>
 hipError_t launch_batch_stage(int stage, int limiter, bool nonideal, const BatchTables& tb,

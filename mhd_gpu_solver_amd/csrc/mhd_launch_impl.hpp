@@ -159,6 +159,22 @@ hipError_t launch_totals(const R* U, const Geom& g, DD* partial, double* out, hi
     return hipGetLastError();
 }
 
+template <typename R>
+int stage_blocks_per_cu(int limiter, bool nonideal, bool remap)
+{
+    int n = 0;
+    hipError_t e;
+    auto q = [&](auto kern) { return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, kBlock, 0); };
+    if (limiter == kMinmod) {
+        e = nonideal ? (remap ? q(k_stage<2, kMinmod, 1, 1, R>) : q(k_stage<2, kMinmod, 1, 0, R>))
+                     : (remap ? q(k_stage<2, kMinmod, 0, 1, R>) : q(k_stage<2, kMinmod, 0, 0, R>));
+    } else {
+        e = nonideal ? (remap ? q(k_stage<2, kMC, 1, 1, R>) : q(k_stage<2, kMC, 1, 0, R>))
+                     : (remap ? q(k_stage<2, kMC, 0, 1, R>) : q(k_stage<2, kMC, 0, 0, R>));
+    }
+    return (e == hipSuccess && n > 0) ? n : 1;
+}
+
 #define MHDHIP_INSTANTIATE(R)                                                                    \
     template hipError_t launch_ctrl_init<R>(Ctrl*, hipStream_t);                                 \
     template hipError_t launch_cfl<R>(const R*, const Geom&, const Physics&, const uint8_t*,     \
@@ -171,6 +187,7 @@ hipError_t launch_totals(const R* U, const Geom& g, DD* partial, double* out, hi
     template hipError_t launch_batch_stage<R>(int, int, bool, const BatchTables&, int, int,       \
                                               hipStream_t);                                       \
     template int batch_blocks_per_cu<R>(int, bool);                                              \
+    template int stage_blocks_per_cu<R>(int, bool, bool);                                        \
     template hipError_t launch_totals<R>(const R*, const Geom&, DD*, double*, hipStream_t);
 
 }  // namespace mhdhip

@@ -458,10 +458,30 @@ struct mhd_hip_solver {
         hipEvent_t e0, e1;
         check(hipEventCreate(&e0), "hipEventCreate");
         check(hipEventCreate(&e1), "hipEventCreate");
+        // candidates: fixed lengths plus the lengths that make the wave count just fill
+        // k whole rounds of the device's concurrent wave slots (no half-empty last round)
+        std::vector<int> cands = {8, 16, 24, 32, 48, 64, 96, 128};
+        {
+            hipDeviceProp_t prop;
+            check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+            const int per_cu = mhdhip::stage_blocks_per_cu<R>(limiter, ni, remap);
+            const long long slots =
+                static_cast<long long>(per_cu) * mhdhip::kWavesPerBlock * prop.multiProcessorCount;
+            const long long strips = (grid.nx + mhdhip::kStripCells - 1) / mhdhip::kStripCells;
+            for (int k = 1; k <= 6; ++k) {
+                const long long segs = std::max<long long>(1, k * slots / strips);
+                const int rows = static_cast<int>((grid.ny + segs - 1) / segs);
+                if (rows >= 4 && rows <= 512) {
+                    cands.push_back(rows);
+                }
+            }
+            std::sort(cands.begin(), cands.end());
+            cands.erase(std::unique(cands.begin(), cands.end()), cands.end());
+        }
         int best = mhdhip::default_seg_rows(grid.nx, grid.ny);
         float best_ms = 1e30f;
         try {
-            for (int cand : {8, 16, 24, 32, 48, 64, 96, 128}) {
+            for (int cand : cands) {
                 if (cand > grid.ny && cand != 8) {
                     continue;
                 }
