@@ -198,10 +198,13 @@ __device__ __forceinline__ void hll_face(R (&WL)[kNC], R (&WR)[kNC], const R (&W
     const R inf = Lit<R>::inf();
     const bool ok = (WL[RHO] > zero && WL[PR] > zero && WL[RHO] < inf && WL[PR] < inf) &&
                     (WR[RHO] > zero && WR[PR] > zero && WR[RHO] < inf && WR[PR] < inf);
+    // the fallback is rare: skip its selects for the whole wave unless some lane needs it
+    if (__builtin_amdgcn_ballot_w64(!ok) != 0) {
 #pragma unroll
-    for (int c = 0; c < kNC; ++c) {
-        WL[c] = ok ? WL[c] : Wm[c];
-        WR[c] = ok ? WR[c] : W0[c];
+        for (int c = 0; c < kNC; ++c) {
+            WL[c] = ok ? WL[c] : Wm[c];
+            WR[c] = ok ? WR[c] : W0[c];
+        }
     }
     const R cfL = fast_speed<DIR>(WL, gamma);
     const R cfR = fast_speed<DIR>(WR, gamma);
@@ -223,8 +226,15 @@ __device__ __forceinline__ void hll_face(R (&WL)[kNC], R (&WR)[kNC], const R (&W
 #pragma unroll
     for (int c = 0; c < kNC; ++c) {
         const R hll = (sR * FL[c] - sL * FR[c] + sLsR * (UR[c] - UL[c])) * inv_denom;
-        const R mid = degenerate ? half * (FL[c] + FR[c]) : hll;
-        F[c] = use_left ? FL[c] : (use_right ? FR[c] : mid);
+        F[c] = use_left ? FL[c] : (use_right ? FR[c] : hll);
+    }
+    // sR - sL spans at least 2 c_h, so the degenerate average is rare: wave-uniform skip
+    if (__builtin_amdgcn_ballot_w64(degenerate && !use_left && !use_right) != 0) {
+#pragma unroll
+        for (int c = 0; c < kNC; ++c) {
+            const R mid = half * (FL[c] + FR[c]);
+            F[c] = (degenerate && !use_left && !use_right) ? mid : F[c];
+        }
     }
 }
 
