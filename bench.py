@@ -128,8 +128,9 @@ def roofline(kernels, cells, ms_per_step, elem_bytes, name, traffic=None):
 # ---- ensemble -------------------------------------------------------------------------------
 
 def ensemble_batched(m, specs, states, precision, steps, warmup, seg_rows=0):
-    """The whole ensemble in one batch engine: `warmup` steps (autotune, graph capture), then
-    `steps` steps timed with events around every stage kernel, then the same on the graph path."""
+    """The whole ensemble in one batch engine: `warmup` steps on each launch path (eager with
+    events, then the graph: tiling, capture), then `steps` steps timed with events around every
+    stage kernel, then the same on the graph path."""
     budget = sum(m.instance_bytes(precision, s.n, s.n) for s in specs) + (64 << 20)
     eng = m.BatchEngine(precision, limiter=m.SlopeLimiter.Minmod, hbm_budget=budget,
                         max_instances=len(specs))
@@ -139,10 +140,12 @@ def ensemble_batched(m, specs, states, precision, steps, warmup, seg_rows=0):
     def admit(k):
         return [eng.add_steps(U, k, gamma=s.gamma, bc=s.bc) for s, U in zip(specs, states)]
 
-    ids = admit(warmup)
-    eng.advance()
-    for i in ids:
-        eng.remove(i)
+    for timing in (True, False):
+        ids = admit(warmup)
+        eng.set_kernel_timing(timing)
+        eng.advance()
+        for i in ids:
+            eng.remove(i)
     ids = admit(steps)
     before = {i: eng.totals(i) for i, s in zip(ids, specs) if s.bc == (0, 0)}
     eng.set_kernel_timing(True)
@@ -278,6 +281,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["ensemble", "single", "serve"], default=None)
     ap.add_argument("--serve-requests", type=int, default=128)
+    ap.add_argument("--batch-rows", type=int, default=0,
+                    help="batch rows per wave: 0 load model, -1 timed once, >0 fixed")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="also run plain device copies of BYTES (counter scale for profiling)")
@@ -303,11 +308,13 @@ def main():
         log(f"ensemble: {len(specs)} instances, {sum(s.n * s.n for s in specs)} cells")
         states = [s.state() for s in specs]
         log(f"batched fp32: {args.warmup} warmup + {args.steps} timed steps")
-        b32 = ensemble_batched(m, specs, states, "fp32", args.steps, args.warmup)
+        b32 = ensemble_batched(m, specs, states, "fp32", args.steps, args.warmup,
+                               args.batch_rows)
         log(f"batched fp32: {b32['mcups']:.1f} Mcell-updates/s ({b32['ms_per_step']:.3f} ms/step,"
             f" graph path {b32['graph_mcups']:.1f})")
         log("batched fp64")
-        b64 = ensemble_batched(m, specs, states, "fp64", args.steps, args.warmup)
+        b64 = ensemble_batched(m, specs, states, "fp64", args.steps, args.warmup,
+                               args.batch_rows)
         log(f"batched fp64: {b64['mcups']:.1f} Mcell-updates/s")
         plain = None
         l1 = None
@@ -389,13 +396,19 @@ def main():
         from mhd_gpu_solver_amd.serve import Server
         reqs = workloads.request_stream(args.serve_requests, rate=200.0, seed=7)
         log(f"serving: {len(reqs)} requests, Poisson 200/s, N~U{{64..1024}}, t_end~U[0.05,0.5]")
-        srv = Server("fp32", hbm_budget=32 << 30, max_instances=1024)
-        rep = srv.run(reqs, realtime=True)
-        summ = rep.summary()
-        log(f"serving: p50 {summ['p50_ms']} ms, p99 {summ['p99_ms']} ms, "
-            f"{summ['mcups']} Mcell-updates/s, max concurrent {summ['max_concurrent']}")
+        runs = {}
+        for policy in ("srpt", "all"):
+            srv = Server("fp32", hbm_budget=32 << 30, max_instances=1024, policy=policy)
+            summ = srv.run(reqs, realtime=True).summary()
+            del srv
+            log(f"serving ({policy}): p50 {summ['p50_ms']} ms, p99 {summ['p99_ms']} ms, "
+                f"{summ['mcups']} Mcell-updates/s, max concurrent {summ['max_concurrent']}")
+            runs[policy] = {"policy": policy, **summ}
+        summ = runs["srpt"]
         out["serving"] = {"workload": "request_stream_poisson_200", "precision": "fp32",
-                          "limiter": "mc", "hbm_budget_bytes": 32 << 30, **summ}
+                          "limiter": "mc", "hbm_budget_bytes": 32 << 30,
+                          "work_cap_cells": Server.DEFAULT_WORK_CAP, **summ,
+                          "processor_sharing": runs["all"]}
         if args.only == "serve":
             out.update({"value": summ["mcups"], "dtype": "f32", "config": {
                 "workload": "request_stream_poisson_200"}, "roofline": None,

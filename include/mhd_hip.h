@@ -203,12 +203,13 @@ typedef struct mhd_hip_instance {
 
 typedef struct mhd_hip_batch_stats {
     int instances; /* admitted (running or finished, not yet removed) */
-    int active;    /* still stepping */
+    int active;    /* still stepping (unfinished, not paused) */
     int tiles;     /* wave work items in the current tile list */
     int seg_rows;  /* rows per wave in use (autotuned unless set) */
     int blocks;    /* workgroups of the persistent stage kernel */
     size_t bytes_used, bytes_budget;
     double last_seconds; /* device time of the last mhd_hip_batch_advance */
+    int paused;          /* resident, unfinished, excluded from stepping */
 } mhd_hip_batch_stats;
 
 /* HBM bytes one instance of an nx x ny grid occupies in the arena (U and U*). */
@@ -239,10 +240,35 @@ int mhd_hip_batch_result(mhd_hip_batch* b, int id, mhd_hip_result* result, int* 
 /* Ghost fill (the solve's final fill) and copy of the instance's state to nine fp64 planes. */
 int mhd_hip_batch_download(mhd_hip_batch* b, int id, double* const comps[9], size_t n);
 
+/* Scheduling hooks. A paused instance stays resident (state and control block untouched) but
+ * is left out of the step graph's tile list from the next replay on; mhd_hip_batch_advance
+ * returns once no unpaused instance is stepping. mhd_hip_batch_progress reports t and steps
+ * as of the last replay and an estimate of the instance's dt (t / steps, or before its first
+ * step the dt of its admission CFL reduction). */
+int mhd_hip_batch_set_paused(mhd_hip_batch* b, int id, int paused);
+int mhd_hip_batch_progress(mhd_hip_batch* b, int id, double* t, int* steps, double* dt_estimate);
+
+/* Engine-side scheduling, applied before every replay. MHD_HIP_SCHED_ALL steps every
+ * unfinished, unpaused instance (the default). MHD_HIP_SCHED_SRPT orders the unfinished
+ * instances by estimated remaining work (interior cells x remaining steps, the steps from the
+ * dt estimate above and capped by max_steps) and steps the least-work ones until
+ * work_cap_cells interior cells are stepping; it sets the paused flags itself, overriding
+ * mhd_hip_batch_set_paused. Scheduling changes when a solve is computed, never its result. */
+#define MHD_HIP_SCHED_ALL 0
+#define MHD_HIP_SCHED_SRPT 1
+int mhd_hip_batch_set_schedule(mhd_hip_batch* b, int policy, long long work_cap_cells);
+
+/* Ids of the instances that have finished and are still resident (at most cap of them, in
+ * slot order), as of the last replay; *count is the number written. */
+int mhd_hip_batch_poll_finished(mhd_hip_batch* b, int* ids, int cap, int* count);
+
 /* mhd_hip_totals for one instance (interior, double-double, deterministic). */
 int mhd_hip_batch_totals(mhd_hip_batch* b, int id, double sums[9]);
 
-/* Rows per wave for the batch kernel; 0 (default) autotunes on the first advance. */
+/* Rows per wave for the batch kernel. 0 (default): chosen for the current stepping set,
+ * whenever it changes, by replaying the kernel's round-robin tile assignment and minimising
+ * the busiest wave's rows; -1: timed over the candidates once, on the first advance's set;
+ * > 0: fixed. */
 int mhd_hip_batch_set_seg_rows(mhd_hip_batch* b, int rows);
 
 int mhd_hip_batch_info(const mhd_hip_batch* b, mhd_hip_batch_stats* stats);

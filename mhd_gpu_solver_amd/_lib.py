@@ -44,7 +44,8 @@ class Instance(C.Structure):
 class BatchStats(C.Structure):
     _fields_ = [("instances", C.c_int), ("active", C.c_int), ("tiles", C.c_int),
                 ("seg_rows", C.c_int), ("blocks", C.c_int), ("bytes_used", C.c_size_t),
-                ("bytes_budget", C.c_size_t), ("last_seconds", C.c_double)]
+                ("bytes_budget", C.c_size_t), ("last_seconds", C.c_double),
+                ("paused", C.c_int)]
 
 
 DP9 = C.POINTER(C.c_double) * 9
@@ -95,6 +96,12 @@ SIGNATURES = [
     ("mhd_hip_batch_totals", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     ("mhd_hip_batch_set_seg_rows", C.c_int, [C.c_void_p, C.c_int]),
     ("mhd_hip_batch_info", C.c_int, [C.c_void_p, C.POINTER(BatchStats)]),
+    ("mhd_hip_batch_set_paused", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    ("mhd_hip_batch_progress", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double),
+                                         C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    ("mhd_hip_batch_set_schedule", C.c_int, [C.c_void_p, C.c_int, C.c_longlong]),
+    ("mhd_hip_batch_poll_finished", C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.c_int,
+                                              C.POINTER(C.c_int)]),
     ("mhd_hip_batch_set_kernel_timing", C.c_int, [C.c_void_p, C.c_int]),
     ("mhd_hip_batch_kernel_times", C.c_int, [C.c_void_p, C.POINTER(C.c_double),
                                              C.POINTER(C.c_longlong)]),

@@ -44,6 +44,7 @@ class BatchEngine:
         self.handle = h
         self.precision = precision
         self._shape = {}
+        self._ids = (C.c_int * int(max_instances))()
 
     def __del__(self):
         h = getattr(self, "handle", None)
@@ -101,6 +102,35 @@ class BatchEngine:
 
     def set_seg_rows(self, rows: int):
         _lib.check(self._l.mhd_hip_batch_set_seg_rows(self.handle, int(rows)))
+
+    def set_paused(self, iid: int, paused: bool):
+        """Keep an instance resident but out of the stepping set (from the next replay)."""
+        _lib.check(self._l.mhd_hip_batch_set_paused(self.handle, int(iid), int(paused)))
+
+    def progress(self, iid: int):
+        """(t, steps, dt estimate) as of the last replay."""
+        t, n, dt = C.c_double(), C.c_int(), C.c_double()
+        _lib.check(self._l.mhd_hip_batch_progress(self.handle, int(iid), C.byref(t), C.byref(n),
+                                                  C.byref(dt)))
+        return t.value, n.value, dt.value
+
+    SCHEDULES = {"all": 0, "srpt": 1}
+
+    def set_schedule(self, policy: str, work_cap_cells: int = 1):
+        """"all": every unfinished solve steps each replay. "srpt": before each replay the
+        engine steps the least-remaining-work solves until `work_cap_cells` cells step and
+        pauses the rest (it then owns the paused flags)."""
+        if policy not in self.SCHEDULES:
+            raise ValueError(f"unknown scheduling policy {policy!r}")
+        _lib.check(self._l.mhd_hip_batch_set_schedule(self.handle, self.SCHEDULES[policy],
+                                                      int(work_cap_cells)))
+
+    def poll_finished(self):
+        """Ids of the finished solves still resident, as of the last replay."""
+        n = C.c_int()
+        _lib.check(self._l.mhd_hip_batch_poll_finished(self.handle, self._ids, len(self._ids),
+                                                       C.byref(n)))
+        return list(self._ids[:n.value])
 
     def set_kernel_timing(self, enable: bool):
         _lib.check(self._l.mhd_hip_batch_set_kernel_timing(self.handle, int(enable)))

@@ -107,6 +107,7 @@ struct Slot {
     size_t bytes = 0;  // U + U*
     mhd_hip_instance desc{};
     bool finished = false;
+    bool paused = false;  // resident but not stepped (scheduler's choice)
     mhd_hip_result result{};
     int bad_dt = 0;
 };
@@ -154,8 +155,11 @@ struct mhd_hip_batch {
     double* totals_out = nullptr;
 
     bool dirty = true;
-    int seg_rows = 0;  // 0: autotune on the first advance
+    int sched_policy = MHD_HIP_SCHED_ALL;
+    long long sched_cap = 0;  // interior cells stepping per replay under SRPT
+    int seg_rows = 0;  // > 0 fixed; 0 load model at every table build; -1 timed once
     int tuned_rows = 0;
+    int last_rows = 0;  // rows per wave of the current tile list
     int ntiles = 0;
     int blocks = 0;
     double last_seconds = 0.0;
@@ -415,6 +419,12 @@ struct mhd_hip_batch {
         R* u = reinterpret_cast<R*>(static_cast<char*>(arena_base) + s.off);
         check(mhdhip::launch_ctrl_init<R>(d_ctrl + si, stream), "ctrl_init");
         check(mhdhip::launch_cfl<R>(u, g, ph, nullptr, d_ctrl + si, stream), "cfl");
+        // the host copy of this slot's control block, so its progress reads right before the
+        // first replay
+        check(hipMemcpyAsync(h_ctrl + si, d_ctrl + si, sizeof(Ctrl), hipMemcpyDeviceToHost,
+                             stream),
+              "hipMemcpyAsync(ctrl)");
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
     }
 
     int add(const mhd_hip_instance& d, const double* const comps[9])
@@ -487,18 +497,56 @@ struct mhd_hip_batch {
         dirty = true;
     }
 
+    static constexpr int kRowCandidates[] = {8, 12, 16, 20, 24, 28, 32, 40, 48, 64, 96, 128};
+
+    // Rows per wave for the current load. The persistent kernel hands tile t to wave
+    // t mod (waves in the grid); a tile of L rows marches L + 2 rows and loads L + 3, so its
+    // cost is taken as 2 L + 5 half-rows. The model replays that assignment for each
+    // candidate length and keeps the one whose busiest wave has the least work, the longer
+    // length on ties (fewer halo re-reads).
+    int model_rows() const
+    {
+        const int waves = std::max(1, blocks * mhdhip::kWavesPerBlock);
+        std::vector<long long> load(static_cast<size_t>(waves));
+        int best = kRowCandidates[0];
+        long long best_cost = std::numeric_limits<long long>::max();
+        for (const int rows : kRowCandidates) {
+            std::fill(load.begin(), load.end(), 0ll);
+            long long k = 0;
+            for (const auto& s : slots) {
+                if (!s.used || s.finished || s.paused) {
+                    continue;
+                }
+                const int strips = (s.desc.nx + mhdhip::kStripCells - 1) / mhdhip::kStripCells;
+                for (int r0 = 0; r0 < s.desc.ny; r0 += rows) {
+                    const long long c = 2ll * std::min(rows, s.desc.ny - r0) + 5;
+                    for (int st = 0; st < strips; ++st, ++k) {
+                        load[static_cast<size_t>(k % waves)] += c;
+                    }
+                }
+            }
+            const long long cost = *std::max_element(load.begin(), load.end());
+            if (cost <= best_cost) {
+                best_cost = cost;
+                best = rows;
+            }
+        }
+        return best;
+    }
+
     int rows_for_tiles() const
     {
         if (seg_rows > 0) return seg_rows;
-        if (tuned_rows > 0) return tuned_rows;
-        return 32;
+        if (seg_rows < 0 && tuned_rows > 0) return tuned_rows;
+        return model_rows();
     }
 
     void build_tables(int rows)
     {
+        last_rows = rows;
         size_t n = 0;
         for (const auto& s : slots) {
-            if (s.used && !s.finished) {
+            if (s.used && !s.finished && !s.paused) {
                 const int strips = (s.desc.nx + mhdhip::kStripCells - 1) / mhdhip::kStripCells;
                 const int segs = (s.desc.ny + rows - 1) / rows;
                 n += static_cast<size_t>(strips) * segs;
@@ -520,7 +568,7 @@ struct mhd_hip_batch {
             in.g = geom_of(s.desc);
             in.ph = physics_of(s.desc);
             in.enabled = 1;
-            if (s.finished) {
+            if (s.finished || s.paused) {
                 continue;
             }
             const int strips = (s.desc.nx + mhdhip::kStripCells - 1) / mhdhip::kStripCells;
@@ -582,7 +630,7 @@ struct mhd_hip_batch {
     {
         int best = 32;
         float best_ms = 1e30f;
-        for (int rows : {8, 12, 16, 20, 24, 28, 32, 40, 48, 64, 96, 128}) {
+        for (const int rows : kRowCandidates) {
             build_tables(rows);
             check(mhdhip::launch_batch_stage<R>(1, limiter, nonideal, tables(), 0, blocks, stream),
                   "tune");
@@ -631,6 +679,62 @@ struct mhd_hip_batch {
         }
     }
 
+    // dt of the next step as the host can estimate it: the mean dt so far, or before the
+    // first step the admission CFL reduction's (include/mhd/cfl.hpp:77-89)
+    double dt_estimate(const Slot& s, const Ctrl& c) const
+    {
+        if (c.steps[0] > 0) {
+            return c.t[0] / c.steps[0];
+        }
+        const Physics ph = physics_of(s.desc);
+        double ch;
+        const unsigned long long bits = c.chmax[0];
+        std::memcpy(&ch, &bits, sizeof(ch));
+        double dt = ch > 0.0 ? ph.cfl / (ch * ph.inv_dx + ch * ph.inv_dy)
+                             : std::numeric_limits<double>::infinity();
+        if (ph.has_diff) dt = std::min(dt, ph.dt_diff);
+        return dt;
+    }
+
+    void schedule()
+    {
+        if (sched_policy != MHD_HIP_SCHED_SRPT) {
+            return;
+        }
+        struct Work {
+            double w;
+            int si;
+            long long cells;
+        };
+        std::vector<Work> order;
+        order.reserve(slots.size());
+        for (int si = 0; si < max_instances; ++si) {
+            const Slot& s = slots[si];
+            if (!s.used || s.finished) {
+                continue;
+            }
+            const Ctrl& c = h_ctrl[si];
+            const double dt = dt_estimate(s, c);
+            const double left = std::max(s.desc.t_end - c.t[0], 0.0);
+            double steps = (dt > 0.0 && std::isfinite(dt)) ? left / dt : 0.0;
+            steps = std::min(steps, static_cast<double>(s.desc.max_steps - c.steps[0]));
+            const long long cells = static_cast<long long>(s.desc.nx) * s.desc.ny;
+            order.push_back(Work{static_cast<double>(cells) * steps, si, cells});
+        }
+        std::stable_sort(order.begin(), order.end(),
+                         [](const Work& a, const Work& b) { return a.w < b.w; });
+        long long stepping = 0;
+        for (const Work& o : order) {
+            const bool run = stepping < sched_cap;
+            stepping += run ? o.cells : 0;
+            Slot& s = slots[o.si];
+            if (s.paused == run) {
+                s.paused = !run;
+                dirty = true;
+            }
+        }
+    }
+
     template <typename R>
     int advance_t(int max_replays)
     {
@@ -639,14 +743,15 @@ struct mhd_hip_batch {
         int active = 0;
         for (int rep = 0; max_replays < 0 || rep < max_replays; ++rep) {
             read_ctrl();
+            schedule();
             active = 0;
             for (const auto& s : slots) {
-                active += (s.used && !s.finished) ? 1 : 0;
+                active += (s.used && !s.finished && !s.paused) ? 1 : 0;
             }
             if (active == 0) {
                 break;
             }
-            if (seg_rows == 0 && tuned_rows == 0) {
+            if (seg_rows < 0 && tuned_rows == 0) {
                 autotune<R>();
             }
             if (dirty) {
@@ -668,7 +773,7 @@ struct mhd_hip_batch {
         read_ctrl();
         active = 0;
         for (const auto& s : slots) {
-            active += (s.used && !s.finished) ? 1 : 0;
+            active += (s.used && !s.finished && !s.paused) ? 1 : 0;
         }
         return active;
     }
@@ -837,9 +942,69 @@ int mhd_hip_batch_set_seg_rows(mhd_hip_batch* b, int rows)
 {
     return guarded([&] {
         if (!b) throw InvalidArgument("null batch");
-        if (rows < 0) throw InvalidArgument("rows per wave segment must be >= 0");
+        if (rows < -1) throw InvalidArgument("rows per wave segment must be >= -1");
         b->seg_rows = rows;
+        b->tuned_rows = 0;
         b->dirty = true;
+    });
+}
+
+int mhd_hip_batch_set_paused(mhd_hip_batch* b, int id, int paused)
+{
+    return guarded([&] {
+        if (!b) throw InvalidArgument("null batch");
+        Slot& s = b->slot_of(id);
+        if (s.paused != (paused != 0)) {
+            s.paused = paused != 0;
+            b->dirty = true;
+        }
+    });
+}
+
+int mhd_hip_batch_progress(mhd_hip_batch* b, int id, double* t, int* steps, double* dt_estimate)
+{
+    return guarded([&] {
+        if (!b || !t || !steps || !dt_estimate) throw InvalidArgument("null argument");
+        const Slot& s = b->slot_of(id);
+        const Ctrl& c = b->h_ctrl[b->slot_index(s)];
+        *t = c.t[0];
+        *steps = c.steps[0];
+        *dt_estimate = b->dt_estimate(s, c);
+    });
+}
+
+int mhd_hip_batch_set_schedule(mhd_hip_batch* b, int policy, long long work_cap_cells)
+{
+    return guarded([&] {
+        if (!b) throw InvalidArgument("null batch");
+        if (policy == MHD_HIP_SCHED_ALL) {
+            for (auto& s : b->slots) {
+                if (s.paused) {
+                    s.paused = false;
+                    b->dirty = true;
+                }
+            }
+        } else if (policy == MHD_HIP_SCHED_SRPT) {
+            if (work_cap_cells < 1) throw InvalidArgument("work cap must be at least one cell");
+        } else {
+            throw InvalidArgument("unknown scheduling policy " + std::to_string(policy));
+        }
+        b->sched_policy = policy;
+        b->sched_cap = work_cap_cells;
+    });
+}
+
+int mhd_hip_batch_poll_finished(mhd_hip_batch* b, int* ids, int cap, int* count)
+{
+    return guarded([&] {
+        if (!b || !count || (cap > 0 && !ids)) throw InvalidArgument("null argument");
+        int n = 0;
+        for (const auto& s : b->slots) {
+            if (s.used && s.finished && n < cap) {
+                ids[n++] = s.id;
+            }
+        }
+        *count = n;
     });
 }
 
@@ -873,10 +1038,15 @@ int mhd_hip_batch_info(const mhd_hip_batch* b, mhd_hip_batch_stats* st)
             n += s.used ? 1 : 0;
             act += (s.used && !s.finished) ? 1 : 0;
         }
+        int paused = 0;
+        for (const auto& s : b->slots) {
+            paused += (s.used && !s.finished && s.paused) ? 1 : 0;
+        }
         st->instances = n;
-        st->active = act;
+        st->active = act - paused;
+        st->paused = paused;
         st->tiles = b->ntiles;
-        st->seg_rows = b->seg_rows > 0 ? b->seg_rows : b->tuned_rows;
+        st->seg_rows = b->last_rows;
         st->blocks = b->blocks;
         st->bytes_used = b->arena.used();
         st->bytes_budget = b->arena.capacity();

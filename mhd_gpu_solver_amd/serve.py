@@ -7,6 +7,12 @@ time; finished solves are downloaded, released and their latency (completion - a
 recorded. New requests join between replays while the others keep running, so their work
 overlaps on the device. No counterpart exists in the reference, which solves one grid per
 call (include/mhd/solver_api.hpp:294-321).
+
+Scheduling: with ``policy="all"`` every admitted solve steps in every replay (processor
+sharing). With ``policy="srpt"`` admitted solves stay resident in HBM but each replay steps
+only those with the least estimated remaining work (cells x remaining steps, the steps taken
+from the device's own dt so far) until ``work_cap_cells`` cells are stepping; the rest are
+paused. Pausing never changes a solve's result, only when it is computed.
 """
 from __future__ import annotations
 
@@ -68,14 +74,24 @@ class ServeReport:
 
 
 class Server:
+    # cells stepping per replay under srpt: about three rounds of a full persistent grid on
+    # MI355X at 32 rows per wave (256 CUs x 3 blocks x 4 waves x 60 x 32 cells = 5.9M)
+    DEFAULT_WORK_CAP = 16 << 20
+
     def __init__(self, precision="fp32", limiter=SlopeLimiter.MC, hbm_budget=32 << 30,
-                 max_instances=1024, cfl=0.4, glm_alpha=0.1):
+                 max_instances=1024, cfl=0.4, glm_alpha=0.1, policy="srpt",
+                 work_cap_cells=DEFAULT_WORK_CAP):
+        if policy not in ("all", "srpt"):
+            raise ValueError(f"unknown scheduling policy {policy!r}")
         self.engine = BatchEngine(precision, limiter=limiter, hbm_budget=hbm_budget,
                                   max_instances=max_instances)
         self.precision = precision
         self.cfl = cfl
         self.glm_alpha = glm_alpha
         self.hbm_budget = hbm_budget
+        self.policy = policy
+        self.work_cap_cells = int(work_cap_cells)
+        self.engine.set_schedule(policy, self.work_cap_cells)
 
     def run(self, requests, realtime=True, keep_states=False, clock=time.perf_counter):
         """Serve `requests` (workloads.Spec with .arrival seconds, .t_end) to completion.
@@ -112,10 +128,8 @@ class Server:
                 continue
             self.engine.advance(max_replays=1)
             replays += 1
-            for iid in list(active):
-                res, fin = self.engine.result(iid)
-                if not fin:
-                    continue
+            for iid in self.engine.poll_finished():
+                res, _ = self.engine.result(iid)
                 k, adm = active.pop(iid)
                 state = self.engine.download(iid) if keep_states else None
                 self.engine.remove(iid)

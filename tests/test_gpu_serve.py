@@ -24,13 +24,15 @@ def alone(hip, req, precision="fp32"):
 def test_served_results_equal_alone(hip, budget_instances):
     from mhd_gpu_solver_amd.serve import Server
     reqs = small_stream()
-    per = max(hip.instance_bytes("fp32", r.n, r.n) for r in reqs)
-    srv = Server("fp32", hbm_budget=budget_instances * per, max_instances=64)
+    sizes = [hip.instance_bytes("fp32", r.n, r.n) for r in reqs]
+    budget = budget_instances * max(sizes)
+    srv = Server("fp32", hbm_budget=budget, max_instances=64)
     rep = srv.run(reqs, realtime=True, keep_states=True)
     assert len(rep.completed) == len(reqs)
-    assert rep.max_concurrent <= budget_instances
+    # first-fit admission: never more resident than the budget holds of the smallest solve
+    assert rep.max_concurrent <= budget // min(sizes)
     if budget_instances == 2:
-        assert rep.max_concurrent == 2
+        assert rep.max_concurrent >= 2
     for c in rep.completed:
         ref, r = alone(hip, reqs[c.index])
         assert c.steps == r.steps and c.t == r.t
@@ -40,11 +42,83 @@ def test_served_results_equal_alone(hip, budget_instances):
     assert s["p99_ms"] >= s["p50_ms"] > 0
 
 
+@pytest.mark.parametrize("cap_cells", [1, 3 * 48 * 48])
+def test_srpt_served_results_equal_alone(hip, cap_cells):
+    """Shortest-remaining-work scheduling pauses resident solves; results stay identical."""
+    from mhd_gpu_solver_amd.serve import Server
+    reqs = small_stream(seed=5, count=8)
+    srv = Server("fp32", hbm_budget=1 << 30, max_instances=16, policy="srpt",
+                 work_cap_cells=cap_cells)
+    rep = srv.run(reqs, realtime=False, keep_states=True)
+    assert len(rep.completed) == len(reqs)
+    for c in rep.completed:
+        ref, r = alone(hip, reqs[c.index])
+        assert c.steps == r.steps and c.t == r.t
+        assert np.array_equal(c.state, ref), (c.index, reqs[c.index].ic)
+    if cap_cells == 1:
+        # one solve steps at a time: completions come one by one in distinct replays
+        fin = [c.finished for c in rep.completed]
+        assert fin == sorted(fin)
+        assert rep.replays >= len(reqs)
+
+
+def test_pause_resume_keeps_state_and_result(hip):
+    reqs = small_stream(seed=6, count=2)
+    eng = hip.BatchEngine("fp32", limiter=hip.SlopeLimiter.MC, hbm_budget=1 << 28)
+    a = eng.add(reqs[0].state(), reqs[0].t_end, gamma=reqs[0].gamma, bc=reqs[0].bc)
+    b = eng.add(reqs[1].state(), reqs[1].t_end, gamma=reqs[1].gamma, bc=reqs[1].bc)
+    t0, n0, dt0 = eng.progress(b)
+    assert t0 == 0.0 and n0 == 0 and 0.0 < dt0 < 1.0
+    eng.set_paused(b, True)
+    eng.advance(max_replays=1)
+    assert eng.progress(b)[:2] == (0.0, 0)
+    assert eng.progress(a)[1] > 0
+    assert eng.stats()["paused"] == 1
+    state_b = eng.download(b)
+    assert np.array_equal(state_b, _ghosted(reqs[1]))
+    eng.advance()  # returns once only paused solves are left
+    assert eng.result(a)[1] and not eng.result(b)[1]
+    eng.set_paused(b, False)
+    eng.advance()
+    for i, req in ((a, reqs[0]), (b, reqs[1])):
+        ref, r = alone(hip, req)
+        assert eng.result(i)[0].steps == r.steps
+        assert np.array_equal(eng.download(i), ref)
+
+
+def test_srpt_steps_least_remaining_work_first(hip):
+    """Under a one-cell cap only the solve with the least estimated work steps."""
+    from mhd_gpu_solver_amd import workloads
+    eng = hip.BatchEngine("fp32", limiter=hip.SlopeLimiter.MC, hbm_budget=1 << 28)
+    specs = [workloads.Spec("alfven_oblique", n, 5.0 / 3.0, 1.0, (0, 0), 0.02)
+             for n in (48, 32, 64)]
+    ids = [eng.add(s.state(), s.t_end, gamma=s.gamma, bc=s.bc) for s in specs]
+    eng.set_schedule("srpt", 1)
+    eng.advance(max_replays=1)
+    steps = [eng.progress(i)[1] for i in ids]
+    assert steps[1] > 0 and steps[0] == 0 and steps[2] == 0
+    assert eng.stats()["paused"] == 2
+    eng.advance()
+    assert sorted(eng.poll_finished()) == sorted(ids)
+    with pytest.raises(ValueError):
+        eng.set_schedule("srpt", 0)
+    eng.set_schedule("all")
+    assert eng.stats()["paused"] == 0
+
+
+def _ghosted(req):
+    """The initial state as download() returns it: ghosts refilled from the interior."""
+    from checkers import Oracle, make_grid
+    U = req.state().copy()
+    Oracle().apply_bc(make_grid(req.n, req.n), U, req.bc)
+    return U.astype(np.float32).astype(np.float64)
+
+
 def test_served_fp64_matches_oracle(hip, oracle):
     from checkers import MC, make_grid, make_params
     from mhd_gpu_solver_amd.serve import Server
     reqs = small_stream(seed=4, count=4)
-    srv = Server("fp64", hbm_budget=1 << 30, max_instances=8)
+    srv = Server("fp64", hbm_budget=1 << 30, max_instances=8, policy="all")
     rep = srv.run(reqs, realtime=False, keep_states=True)
     for c in rep.completed:
         req = reqs[c.index]

@@ -126,3 +126,9 @@ def test_null_and_bad_arguments_return_codes():
     assert lib.mhd_hip_create(C.byref(g), 0, C.byref(h)) in (_lib.EINVAL, _lib.ENODEV)
     assert lib.mhd_hip_solve(None, None, None) == _lib.EINVAL
     assert lib.mhd_hip_last_error()
+
+
+def test_server_rejects_unknown_policy():
+    from mhd_gpu_solver_amd.serve import Server
+    with pytest.raises(ValueError):
+        Server(policy="lifo")
