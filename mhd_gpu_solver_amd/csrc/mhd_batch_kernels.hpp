@@ -38,7 +38,7 @@ template <int STAGE, int LIM, int NI, typename R>
 __global__ __launch_bounds__(kBlock) void k_batch_stage(BatchTables tb, int parity)
 {
     const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x >> 6;
+    const int wave = wave_in_block();
     const int gw = xcd_logical_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
     const int nw = gridDim.x * kWavesPerBlock;
     const int n = *tb.ntiles;

@@ -30,6 +30,14 @@ constexpr int kStripCells = kWave - 4;  // cells per wave strip: 2 halo lanes le
 constexpr int kWavesPerBlock = 4;
 constexpr int kBlock = kWave * kWavesPerBlock;
 
+// Index of the calling wave in its workgroup. Lanes 64w..64w+63 form wave w, so the value is
+// the same on every lane; reading it through readfirstlane tells the compiler so, and what is
+// derived from it (strip, rows, row addresses, loop bounds) stays in scalar registers.
+__device__ __forceinline__ int wave_in_block()
+{
+    return __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+}
+
 struct Ctrl {
     double t[2];
     double ch_last[2];
@@ -640,7 +648,7 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageBuffers b, Geom g, Physic
     // Consecutive logical blocks (neighbouring strips) are placed on the same XCD so that the
     // halo columns and the cache lines shared across a strip edge meet in one L2.
     const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x >> 6;
+    const int wave = wave_in_block();
     const int logical = xcd_logical_block(blockIdx.x, gridDim.x);
     const int strip = logical % tl.strips;
     const int seg = (logical / tl.strips) * kWavesPerBlock + wave;
