@@ -38,6 +38,41 @@ __device__ __forceinline__ int wave_in_block()
     return __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
 }
 
+// Neighbour exchange across the wave through the whole-wave DPP shifts of GFX9 (wave_shr:1,
+// wave_shl:1): one VALU move instead of an LDS permute. from_left(x) on lane i is x of lane
+// i - 1, from_right(x) x of lane i + 1; the edge lane with no source keeps its own x, as
+// __shfl_up(x, 1) / __shfl_down(x, 1) do. Callers keep all 64 lanes active.
+template <int CTRL>
+__device__ __forceinline__ int dpp_move(int x)
+{
+    return __builtin_amdgcn_update_dpp(x, x, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float x)
+{
+    return __builtin_bit_cast(float, dpp_move<CTRL>(__builtin_bit_cast(int, x)));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double x)
+{
+    const unsigned long long v = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = static_cast<unsigned>(dpp_move<CTRL>(static_cast<int>(v & 0xffffffffu)));
+    const unsigned hi = static_cast<unsigned>(dpp_move<CTRL>(static_cast<int>(v >> 32)));
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+constexpr int kDppWaveShr1 = 0x138;
+constexpr int kDppWaveShl1 = 0x130;
+template <typename R>
+__device__ __forceinline__ R from_left(R x)
+{
+    return dpp_move<kDppWaveShr1>(x);
+}
+template <typename R>
+__device__ __forceinline__ R from_right(R x)
+{
+    return dpp_move<kDppWaveShl1>(x);
+}
+
 struct Ctrl {
     double t[2];
     double ch_last[2];
@@ -496,10 +531,10 @@ __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, 
         R Jc[3] = {R(0), R(0), R(0)};
         if (NI) {
             // J = curl B at (gi, c): include/mhd/nonideal.hpp:30-60
-            const R byL = __shfl_up(Wb[BY], 1);
-            const R byR = __shfl_down(Wb[BY], 1);
-            const R bzL = __shfl_up(Wb[BZ], 1);
-            const R bzR = __shfl_down(Wb[BZ], 1);
+            const R byL = from_left(Wb[BY]);
+            const R byR = from_right(Wb[BY]);
+            const R bzL = from_left(Wb[BZ]);
+            const R bzR = from_right(Wb[BZ]);
             const R dbz_dy = (Wn[BZ] - Wa[BZ]) * inv_2dy;
             const R dbz_dx = (bzR - bzL) * inv_2dx;
             const R dby_dx = (byR - byL) * inv_2dx;
@@ -531,11 +566,11 @@ __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, 
                 R Wl[kNC], WRx[kNC], WLx[kNC];
 #pragma unroll
                 for (int k = 0; k < kNC; ++k) {
-                    Wl[k] = __shfl_up(Wa[k], 1);
-                    const R wr = __shfl_down(Wa[k], 1);
+                    Wl[k] = from_left(Wa[k]);
+                    const R wr = from_right(Wa[k]);
                     const R d = limit_slope<LIM>(Wa[k] - Wl[k], wr - Wa[k]);
                     WRx[k] = Wa[k] - half * d;
-                    WLx[k] = __shfl_up(Wa[k] + half * d, 1);
+                    WLx[k] = from_left(Wa[k] + half * d);
                 }
                 R F[kNC];
                 hll_face<0>(WLx, WRx, Wl, Wa, gamma, inv_gm1, c_h, F);
@@ -544,7 +579,7 @@ __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, 
                     R Jf[3], Bf[3], E[3];
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
-                        Jf[k] = half * (__shfl_up(Jp[k], 1) + Jp[k]);
+                        Jf[k] = half * (from_left(Jp[k]) + Jp[k]);
                         Bf[k] = half * (Wl[BX + k] + Wa[BX + k]);
                     }
                     nonideal_e(Jf, Bf, ph.ni, E);
@@ -555,7 +590,7 @@ __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, 
                 R L[kNC];
 #pragma unroll
                 for (int k = 0; k < kNC; ++k) {
-                    const R Fr = __shfl_down(F[k], 1);
+                    const R Fr = from_right(F[k]);
                     // include/mhd/rhs.hpp:116-123
                     L[k] = -(Fr - F[k]) * inv_dx - (G[k] - Gp[k]) * inv_dy;
                 }

@@ -9,10 +9,11 @@
 // compile-time indices of FaceDir<DIR>, which replaces the reference's load/store rotation.
 //
 // Precision policy (template parameter R):
-//   double: everything in fp64.
-//   float : bulk arithmetic in fp32; pressure recovery from total energy and the fast-speed
-//           discriminant are evaluated in fp64 (MixedOps<float>), because both subtract nearly
-//           equal quantities at low plasma beta / strong field.
+//   double: everything in fp64, in the reference's operation order.
+//   float : bulk arithmetic in fp32. Pressure recovery from total energy is evaluated in fp64
+//           (MixedOps<float>), because it subtracts nearly equal quantities at low plasma beta
+//           or high Mach number; the fast-speed discriminant uses a cancellation-free form and
+//           the HLL flux clamped wave speeds (same values up to rounding, fewer instructions).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -118,9 +119,13 @@ struct MixedOps<float> {
                                                      float mz, float e, float bx, float by,
                                                      float bz, float gm1)
     {
-        (void)inv_rho;
         const double dmx = mx, dmy = my, dmz = mz, dbx = bx, dby = by, dbz = bz;
-        const double ekin = 0.5 * (dmx * dmx + dmy * dmy + dmz * dmz) / static_cast<double>(rho);
+        // 1/rho in fp64 from the fp32 reciprocal and one Newton step (relative error ~2^-46),
+        // in place of an fp64 division
+        const double r = rho;
+        double inv = inv_rho;
+        inv = __builtin_fma(__builtin_fma(-r, inv, 1.0), inv, inv);
+        const double ekin = 0.5 * (dmx * dmx + dmy * dmy + dmz * dmz) * inv;
         const double emag = 0.5 * (dbx * dbx + dby * dby + dbz * dbz);
         return static_cast<float>(static_cast<double>(gm1) *
                                   (static_cast<double>(e) - ekin - emag));
@@ -136,6 +141,16 @@ __device__ __forceinline__ R fast_speed(const R (&W)[kNC], R gamma)
     const R inv_rho = drcp(W[RHO]);
     const R cs2 = gamma * W[PR] * inv_rho;
     const R bn2 = W[D::BN] * W[D::BN];
+    if constexpr (sizeof(R) == 4) {
+        // fp32: (cs2 + va2)^2 - 4 cs2 van2 == (cs2 - va2)^2 + 4 cs2 (va2 - van2), and
+        // va2 - van2 = (bt^2 + bz^2) / rho >= 0, so this form has no cancellation
+        const R bp2 = W[D::BT] * W[D::BT] + W[BZ] * W[BZ];
+        const R va2 = (bn2 + bp2) * inv_rho;
+        const R dlt = cs2 - va2;
+        const R disc = dlt * dlt + R(4) * cs2 * (bp2 * inv_rho);
+        const R cf2 = half * (cs2 + va2 + dsqrt(disc));
+        return dsqrt(vmax(cf2, R(0)));
+    }
     const R b2 = bn2 + W[D::BT] * W[D::BT] + W[BZ] * W[BZ];
     const R va2 = b2 * inv_rho;
     const R van2 = bn2 * inv_rho;
@@ -216,12 +231,36 @@ __device__ __forceinline__ void hll_face(R (&WL)[kNC], R (&WR)[kNC], const R (&W
     cons_flux<DIR>(WL, inv_gm1, ch2, UL, FL);
     cons_flux<DIR>(WR, inv_gm1, ch2, UR, FR);
 
-    const R denom = sR - sL;
-    const R inv_denom = drcp(denom);
-    const R sLsR = sL * sR;
     const R half(0.5);
     const bool use_left = sL >= zero;
     const bool use_right = sR <= zero;
+    if constexpr (sizeof(R) == 4) {
+        // fp32: with sL clamped to <= 0 and sR to >= 0 the HLL formula itself returns FL
+        // (sL >= 0) or FR (sR <= 0), to rounding, so no per-component selects are needed
+        const R sLc = vmin(sL, zero);
+        const R sRc = vmax(sR, zero);
+        const R denom = sRc - sLc;
+        const R inv_denom = drcp(denom);
+        const R sLsR = sLc * sRc;
+#pragma unroll
+        for (int c = 0; c < kNC; ++c) {
+            F[c] = (sRc * FL[c] - sLc * FR[c] + sLsR * (UR[c] - UL[c])) * inv_denom;
+        }
+        // sR - sL >= 2 cf > 0 for admissible states; a vanishing span takes the reference's
+        // upwind / average choice (rare: wave-uniform skip)
+        const bool degenerate = denom < Lit<R>::kDegenerate;
+        if (__builtin_amdgcn_ballot_w64(degenerate) != 0) {
+#pragma unroll
+            for (int c = 0; c < kNC; ++c) {
+                const R mid = half * (FL[c] + FR[c]);
+                F[c] = degenerate ? (use_left ? FL[c] : (use_right ? FR[c] : mid)) : F[c];
+            }
+        }
+        return;
+    }
+    const R denom = sR - sL;
+    const R inv_denom = drcp(denom);
+    const R sLsR = sL * sR;
     const bool degenerate = dabs(denom) < Lit<R>::kDegenerate;
 #pragma unroll
     for (int c = 0; c < kNC; ++c) {
