@@ -39,13 +39,14 @@ __device__ __forceinline__ int wave_in_block()
 }
 
 // Neighbour exchange across the wave through the whole-wave DPP shifts of GFX9 (wave_shr:1,
-// wave_shl:1): one VALU move instead of an LDS permute. from_left(x) on lane i is x of lane
-// i - 1, from_right(x) x of lane i + 1; the edge lane with no source keeps its own x, as
-// __shfl_up(x, 1) / __shfl_down(x, 1) do. Callers keep all 64 lanes active.
+// wave_shl:1): one VALU move, often folded into its consumer, instead of an LDS permute.
+// from_left(x) on lane i is x of lane i - 1, from_right(x) x of lane i + 1; the edge lane with
+// no source (lane 0, resp. 63) reads 0. Callers keep all 64 lanes active, and nothing a cell
+// lane stores depends on an edge lane's zero (see kXFaceLanes).
 template <int CTRL>
 __device__ __forceinline__ int dpp_move(int x)
 {
-    return __builtin_amdgcn_update_dpp(x, x, CTRL, 0xF, 0xF, false);
+    return __builtin_amdgcn_mov_dpp(x, CTRL, 0xF, 0xF, true);
 }
 template <int CTRL>
 __device__ __forceinline__ float dpp_move(float x)
@@ -60,6 +61,11 @@ __device__ __forceinline__ double dpp_move(double x)
     const unsigned hi = static_cast<unsigned>(dpp_move<CTRL>(static_cast<int>(v >> 32)));
     return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
 }
+// x-faces whose fluxes cell lanes 2..61 use: the left faces of lanes 2..62. Faces of lanes 0,
+// 1 and 63 see zero-filled neighbours and are never used; wave-uniform branch decisions on
+// x-faces are taken over these lanes only.
+constexpr unsigned long long kXFaceLanes = ((1ull << 61) - 1ull) << 2;
+
 constexpr int kDppWaveShr1 = 0x138;
 constexpr int kDppWaveShl1 = 0x130;
 template <typename R>
@@ -546,7 +552,7 @@ __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, 
         if (c >= r0) {
             // y-face c - 1/2 between rows c-1 (Wa) and c (Wb)
             R G[kNC];
-            hll_face<1>(WLy, WRy, Wa, Wb, gamma, inv_gm1, c_h, G);
+            hll_face<1>(WLy, WRy, Wa, Wb, gamma, inv_gm1, c_h, G, ~0ull);
             if (NI) {
                 // include/mhd/rhs.hpp:96-114
                 R Jf[3], Bf[3], E[3];
@@ -573,7 +579,7 @@ __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, 
                     WLx[k] = from_left(Wa[k] + half * d);
                 }
                 R F[kNC];
-                hll_face<0>(WLx, WRx, Wl, Wa, gamma, inv_gm1, c_h, F);
+                hll_face<0>(WLx, WRx, Wl, Wa, gamma, inv_gm1, c_h, F, kXFaceLanes);
                 if (NI) {
                     // include/mhd/rhs.hpp:74-93
                     R Jf[3], Bf[3], E[3];

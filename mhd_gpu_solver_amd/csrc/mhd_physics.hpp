@@ -203,18 +203,29 @@ __device__ __forceinline__ void cons_flux(const R (&W)[kNC], R inv_gm1, R ch2, R
 // ---- HLL + GLM flux from reconstructed face states: muscl_hll.hpp:145-209 ----------------
 // WL/WR are the MUSCL states; Wm/W0 are the cells left/right of the face, used when either
 // reconstructed state is non-positive or non-finite (both sides then revert to first order).
+// 0 < x < inf (false for NaN): one class test for positive denormal or positive normal
+__device__ __forceinline__ bool positive_finite(double x)
+{
+    return __builtin_amdgcn_class(x, (1 << 7) | (1 << 8));
+}
+__device__ __forceinline__ bool positive_finite(float x)
+{
+    return __builtin_amdgcn_classf(x, (1 << 7) | (1 << 8));
+}
+
+// `live`: the lanes whose flux is used; the rare-path branches below are taken for the whole
+// wave when any live lane needs them.
 template <int DIR, typename R>
 __device__ __forceinline__ void hll_face(R (&WL)[kNC], R (&WR)[kNC], const R (&Wm)[kNC],
                                          const R (&W0)[kNC], R gamma, R inv_gm1, R c_h,
-                                         R (&F)[kNC])
+                                         R (&F)[kNC], unsigned long long live)
 {
     using D = FaceDir<DIR>;
     const R zero(0);
-    const R inf = Lit<R>::inf();
-    const bool ok = (WL[RHO] > zero && WL[PR] > zero && WL[RHO] < inf && WL[PR] < inf) &&
-                    (WR[RHO] > zero && WR[PR] > zero && WR[RHO] < inf && WR[PR] < inf);
+    const bool ok = positive_finite(WL[RHO]) && positive_finite(WL[PR]) &&
+                    positive_finite(WR[RHO]) && positive_finite(WR[PR]);
     // the fallback is rare: skip its selects for the whole wave unless some lane needs it
-    if (__builtin_amdgcn_ballot_w64(!ok) != 0) {
+    if ((__builtin_amdgcn_ballot_w64(!ok) & live) != 0) {
 #pragma unroll
         for (int c = 0; c < kNC; ++c) {
             WL[c] = ok ? WL[c] : Wm[c];
@@ -241,15 +252,17 @@ __device__ __forceinline__ void hll_face(R (&WL)[kNC], R (&WR)[kNC], const R (&W
         const R sRc = vmax(sR, zero);
         const R denom = sRc - sLc;
         const R inv_denom = drcp(denom);
-        const R sLsR = sLc * sRc;
+        const R a = sRc * inv_denom;
+        const R b = -sLc * inv_denom;
+        const R k = sLc * sRc * inv_denom;
 #pragma unroll
         for (int c = 0; c < kNC; ++c) {
-            F[c] = (sRc * FL[c] - sLc * FR[c] + sLsR * (UR[c] - UL[c])) * inv_denom;
+            F[c] = a * FL[c] + (b * FR[c] + k * (UR[c] - UL[c]));
         }
         // sR - sL >= 2 cf > 0 for admissible states; a vanishing span takes the reference's
         // upwind / average choice (rare: wave-uniform skip)
         const bool degenerate = denom < Lit<R>::kDegenerate;
-        if (__builtin_amdgcn_ballot_w64(degenerate) != 0) {
+        if ((__builtin_amdgcn_ballot_w64(degenerate) & live) != 0) {
 #pragma unroll
             for (int c = 0; c < kNC; ++c) {
                 const R mid = half * (FL[c] + FR[c]);
@@ -268,7 +281,7 @@ __device__ __forceinline__ void hll_face(R (&WL)[kNC], R (&WR)[kNC], const R (&W
         F[c] = use_left ? FL[c] : (use_right ? FR[c] : hll);
     }
     // sR - sL spans at least 2 c_h, so the degenerate average is rare: wave-uniform skip
-    if (__builtin_amdgcn_ballot_w64(degenerate && !use_left && !use_right) != 0) {
+    if ((__builtin_amdgcn_ballot_w64(degenerate && !use_left && !use_right) & live) != 0) {
 #pragma unroll
         for (int c = 0; c < kNC; ++c) {
             const R mid = half * (FL[c] + FR[c]);
