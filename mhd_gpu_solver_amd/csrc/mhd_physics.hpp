@@ -213,6 +213,12 @@ __device__ __forceinline__ bool positive_finite(float x)
     return __builtin_amdgcn_classf(x, (1 << 7) | (1 << 8));
 }
 
+// True on every lane when `pred` holds on some lane of `live`.
+__device__ __forceinline__ bool any_lane(bool pred, unsigned long long live)
+{
+    return (__builtin_amdgcn_ballot_w64(pred) & live) != 0;
+}
+
 // `live`: the lanes whose flux is used; the rare-path branches below are taken for the whole
 // wave when any live lane needs them.
 template <int DIR, typename R>
@@ -225,7 +231,7 @@ __device__ __forceinline__ void hll_face(R (&WL)[kNC], R (&WR)[kNC], const R (&W
     const bool ok = positive_finite(WL[RHO]) && positive_finite(WL[PR]) &&
                     positive_finite(WR[RHO]) && positive_finite(WR[PR]);
     // the fallback is rare: skip its selects for the whole wave unless some lane needs it
-    if ((__builtin_amdgcn_ballot_w64(!ok) & live) != 0) {
+    if (any_lane(!ok, live)) {
 #pragma unroll
         for (int c = 0; c < kNC; ++c) {
             WL[c] = ok ? WL[c] : Wm[c];
@@ -246,28 +252,22 @@ __device__ __forceinline__ void hll_face(R (&WL)[kNC], R (&WR)[kNC], const R (&W
     const bool use_left = sL >= zero;
     const bool use_right = sR <= zero;
     if constexpr (sizeof(R) == 4) {
-        // fp32: with sL clamped to <= 0 and sR to >= 0 the HLL formula itself returns FL
-        // (sL >= 0) or FR (sR <= 0), to rounding, so no per-component selects are needed
+        // fp32: with sL clamped to <= 0 and sR to >= the smallest normal float, the HLL formula
+        // itself returns FL (sL >= 0, also when both speeds vanish) or FR (sR <= 0), to
+        // rounding, so no per-component selects are needed. sR - sL >= 2 c_h > 0 for
+        // admissible states, so the reference's average for a span below 1e-14 with
+        // sL < 0 < sR is not reached; such a span would blend FL and FR by the speeds.
+        (void)use_left;
+        (void)use_right;
         const R sLc = vmin(sL, zero);
-        const R sRc = vmax(sR, zero);
-        const R denom = sRc - sLc;
-        const R inv_denom = drcp(denom);
+        const R sRc = vmax(sR, R(1.17549435e-38f));
+        const R inv_denom = drcp(sRc - sLc);
         const R a = sRc * inv_denom;
         const R b = -sLc * inv_denom;
         const R k = sLc * sRc * inv_denom;
 #pragma unroll
         for (int c = 0; c < kNC; ++c) {
             F[c] = a * FL[c] + (b * FR[c] + k * (UR[c] - UL[c]));
-        }
-        // sR - sL >= 2 cf > 0 for admissible states; a vanishing span takes the reference's
-        // upwind / average choice (rare: wave-uniform skip)
-        const bool degenerate = denom < Lit<R>::kDegenerate;
-        if ((__builtin_amdgcn_ballot_w64(degenerate) & live) != 0) {
-#pragma unroll
-            for (int c = 0; c < kNC; ++c) {
-                const R mid = half * (FL[c] + FR[c]);
-                F[c] = degenerate ? (use_left ? FL[c] : (use_right ? FR[c] : mid)) : F[c];
-            }
         }
         return;
     }
@@ -281,7 +281,7 @@ __device__ __forceinline__ void hll_face(R (&WL)[kNC], R (&WR)[kNC], const R (&W
         F[c] = use_left ? FL[c] : (use_right ? FR[c] : hll);
     }
     // sR - sL spans at least 2 c_h, so the degenerate average is rare: wave-uniform skip
-    if ((__builtin_amdgcn_ballot_w64(degenerate && !use_left && !use_right) & live) != 0) {
+    if (any_lane(degenerate && !use_left && !use_right, live)) {
 #pragma unroll
         for (int c = 0; c < kNC; ++c) {
             const R mid = half * (FL[c] + FR[c]);
