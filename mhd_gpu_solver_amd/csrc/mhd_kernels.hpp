@@ -526,6 +526,16 @@ __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, 
         if (c + 2 <= r1 + 1) {
             load_row(c + 2, Uraw);
         }
+        // stage 2 updates row c - 1 from U at the start of the step: issue those loads now so
+        // that their latency overlaps the face computations (column clamped like the row loads)
+        R U0r[kNC];
+        if (STAGE == 2 && c >= r0 + 1) {
+            const long long base = static_cast<long long>(c - 1) * g.nxt + gic;
+#pragma unroll
+            for (int k = 0; k < kNC; ++k) {
+                U0r[k] = u0[k * P + base];
+            }
+        }
         // y slope of row c and the two face states it produces
         R WRy[kNC], WLn[kNC];
 #pragma unroll
@@ -624,7 +634,7 @@ __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, 
                         R un[kNC];
 #pragma unroll
                         for (int k = 0; k < kNC; ++k) {
-                            un[k] = half * (u0[k * P + cell] + ring[slot][k][lane]) +
+                            un[k] = half * (U0r[k] + ring[slot][k][lane]) +
                                     half_dt * L[k];
                         }
 #pragma unroll
