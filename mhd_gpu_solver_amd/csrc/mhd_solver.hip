@@ -428,9 +428,12 @@ struct mhd_hip_solver {
         return n;
     }
 
-    // Autotuner: times the stage-1 kernel (which leaves U untouched) for candidate row-segment
-    // lengths on the current state and keeps the fastest per shape and kernel variant. Runs
-    // after the first CFL reduction of a solve, so the kernels see a live step.
+    // Autotuner: times one step's pair of stage kernels for candidate row-segment lengths on
+    // the current state and keeps the fastest per shape and kernel variant. Stage 1 writes U*
+    // from U; the timed stage 2 reads U* and U and writes U* again, so U is left untouched
+    // (U* is scratch until the next stage 1; the control fields the two launches write are
+    // rewritten by the step's own stages). Runs after the first CFL reduction of a solve, so
+    // the kernels see a live step.
     template <typename R>
     int tuned_rows(const Physics& ph, int limiter, bool ni)
     {
@@ -455,6 +458,7 @@ struct mhd_hip_solver {
         R* u = static_cast<R*>(U);
         R* us = static_cast<R*>(Us);
         StageBuffers b1{u, u, us, masked ? mask_dev : nullptr, ctrl, nullptr, 0};
+        StageBuffers b2{us, u, us, masked ? mask_dev : nullptr, ctrl, nullptr, 0};
         hipEvent_t e0, e1;
         check(hipEventCreate(&e0), "hipEventCreate");
         check(hipEventCreate(&e1), "hipEventCreate");
@@ -486,13 +490,18 @@ struct mhd_hip_solver {
                     continue;
                 }
                 const Tiling tl = mhdhip::make_tiling(grid.nx, grid.ny, cand);
-                // one untimed launch, then the best of three
+                // one untimed pair, then the best of three
                 check(mhdhip::launch_stage<R>(1, limiter, ni, remap, b1, geom, ph, tl, 0, stream),
+                      "tune");
+                check(mhdhip::launch_stage<R>(2, limiter, ni, remap, b2, geom, ph, tl, 0, stream),
                       "tune");
                 float cand_ms = 1e30f;
                 for (int rep = 0; rep < 3; ++rep) {
                     check(hipEventRecord(e0, stream), "hipEventRecord");
                     check(mhdhip::launch_stage<R>(1, limiter, ni, remap, b1, geom, ph, tl, 0,
+                                                  stream),
+                          "tune");
+                    check(mhdhip::launch_stage<R>(2, limiter, ni, remap, b2, geom, ph, tl, 0,
                                                   stream),
                           "tune");
                     check(hipEventRecord(e1, stream), "hipEventRecord");
