@@ -139,6 +139,12 @@ int mhd_hip_fill_boundaries(mhd_hip_solver* s, double gamma, int what);
  * mass/energy drift; the reference sums plainly on the host (diagnostics.hpp:42-57). */
 int mhd_hip_totals(mhd_hip_solver* s, double sums[9]);
 
+/* max |dBx/dx + dBy/dy| over the interior by central differences of the device state, ghosts
+ * included, in fp64 with the reference's operation order and max rule; after a solve (which
+ * ends with the ghost and wall fill) it equals the reference's MHDSolver::max_div_b
+ * (solver_api.hpp:344-347, inititial_condition.hpp:273-287) for the same state. */
+int mhd_hip_max_div_b(mhd_hip_solver* s, double* value);
+
 /* Seconds of device time spent in the last solve/advance call (HIP events on the solver's
  * stream around the step loop, excluding upload/download). */
 int mhd_hip_last_solve_seconds(const mhd_hip_solver* s, double* seconds);
@@ -261,6 +267,9 @@ int mhd_hip_batch_set_schedule(mhd_hip_batch* b, int policy, long long work_cap_
 /* Ids of the instances that have finished and are still resident (at most cap of them, in
  * slot order), as of the last replay; *count is the number written. */
 int mhd_hip_batch_poll_finished(mhd_hip_batch* b, int* ids, int cap, int* count);
+
+/* mhd_hip_max_div_b for one instance, after its final ghost fill (the state download sees). */
+int mhd_hip_batch_max_div_b(mhd_hip_batch* b, int id, double* value);
 
 /* mhd_hip_totals for one instance (interior, double-double, deterministic). */
 int mhd_hip_batch_totals(mhd_hip_batch* b, int id, double sums[9]);

@@ -100,6 +100,12 @@ class BatchEngine:
         _lib.check(self._l.mhd_hip_batch_totals(self.handle, int(iid), out))
         return np.array(out[:])
 
+    def max_div_b(self, iid: int) -> float:
+        """max |div B| of one instance on the device (after its final ghost fill)."""
+        v = C.c_double()
+        _lib.check(self._l.mhd_hip_batch_max_div_b(self.handle, int(iid), C.byref(v)))
+        return v.value
+
     def set_seg_rows(self, rows: int):
         _lib.check(self._l.mhd_hip_batch_set_seg_rows(self.handle, int(rows)))
 

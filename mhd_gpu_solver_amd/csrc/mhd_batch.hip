@@ -832,6 +832,35 @@ struct mhd_hip_batch {
         check(hipStreamSynchronize(stream), "hipStreamSynchronize");
     }
 
+    // max |div B| of one instance after its final ghost fill (as download sees the state)
+    template <typename R>
+    double max_div_b_t(const Slot& s)
+    {
+        const Geom g = geom_of(s.desc);
+        const Physics ph = physics_of(s.desc);
+        R* u = reinterpret_cast<R*>(static_cast<char*>(arena_base) + s.off);
+        const Gate open{d_ctrl, 0.0, 0, -1};
+        check(mhdhip::launch_ghost_fill<R>(u, g, open, stream), "ghost_fill");
+        auto* out = reinterpret_cast<unsigned long long*>(totals_out);
+        check(mhdhip::launch_max_div_b<R>(u, g, ph.inv_2dx, ph.inv_2dy, out, stream), "max_div_b");
+        double v = 0.0;
+        check(hipMemcpyAsync(&v, totals_out, sizeof(double), hipMemcpyDeviceToHost, stream),
+              "hipMemcpyAsync(max_div_b)");
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        return v;
+    }
+
+    double max_div_b(int id)
+    {
+        const Slot& s = slot_of(id);
+        hipSetDevice(device);
+        if (!totals_out) {
+            check(hipMalloc(reinterpret_cast<void**>(&totals_out), 9 * sizeof(double)),
+                  "hipMalloc(totals)");
+        }
+        return precision == MHD_HIP_FP64 ? max_div_b_t<double>(s) : max_div_b_t<float>(s);
+    }
+
     void download(int id, double* const comps[9], size_t n)
     {
         const Slot& s = slot_of(id);
@@ -946,6 +975,14 @@ int mhd_hip_batch_set_seg_rows(mhd_hip_batch* b, int rows)
         b->seg_rows = rows;
         b->tuned_rows = 0;
         b->dirty = true;
+    });
+}
+
+int mhd_hip_batch_max_div_b(mhd_hip_batch* b, int id, double* value)
+{
+    return guarded([&] {
+        if (!b || !value) throw InvalidArgument("null argument");
+        *value = b->max_div_b(id);
     });
 }
 

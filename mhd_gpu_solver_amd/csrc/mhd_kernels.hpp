@@ -417,6 +417,39 @@ __global__ __launch_bounds__(kTotalsBlock) void k_totals_partial(const R* __rest
     }
 }
 
+// max |dBx/dx + dBy/dy| over the interior cells by central differences of the stored state,
+// ghosts included, evaluated in fp64 without contraction:
+// include/initial_condition/inititial_condition.hpp:273-287. The running max follows
+// std::max (a NaN never replaces it); the values are >= 0, so their bit patterns order like
+// the values and one unsigned atomic max per wave combines them in any order. `out` starts at 0.
+template <typename R>
+__global__ __launch_bounds__(kTotalsBlock) void k_max_div_b(const R* __restrict__ U, Geom g,
+                                                            double inv_2dx, double inv_2dy,
+                                                            unsigned long long* out)
+{
+#pragma clang fp contract(off)
+    const R* bx = U + BX * g.plane;
+    const R* by = U + BY * g.plane;
+    const long long n = static_cast<long long>(g.nx) * g.ny;
+    double m = 0.0;
+    for (long long k = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; k < n;
+         k += static_cast<long long>(gridDim.x) * blockDim.x) {
+        const int j = static_cast<int>(k / g.nx);
+        const int i = static_cast<int>(k - static_cast<long long>(j) * g.nx);
+        const long long c = static_cast<long long>(j + g.ng) * g.nxt + (i + g.ng);
+        const double dbx =
+            (static_cast<double>(bx[c + 1]) - static_cast<double>(bx[c - 1])) * inv_2dx;
+        const double dby =
+            (static_cast<double>(by[c + g.nxt]) - static_cast<double>(by[c - g.nxt])) * inv_2dy;
+        const double a = __builtin_fabs(dbx + dby);
+        m = (m < a) ? a : m;
+    }
+    m = wave_max(m);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        atomic_max_speed(out, m);
+    }
+}
+
 // One wave per component: lane l adds partials l, l+64, ... in order, then a fixed shuffle tree.
 template <typename R>
 __global__ __launch_bounds__(kNC * kWave) void k_totals_final(const DD* __restrict__ partial,

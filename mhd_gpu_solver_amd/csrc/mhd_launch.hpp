@@ -37,6 +37,11 @@ constexpr int kTotalsBlocks = 512;
 template <typename R>
 hipError_t launch_totals(const R* U, const Geom& g, DD* partial, double* out, hipStream_t s);
 
+// max |div B| of the interior by central differences (stored ghosts used) into *out.
+template <typename R>
+hipError_t launch_max_div_b(const R* U, const Geom& g, double inv_2dx, double inv_2dy,
+                            unsigned long long* out, hipStream_t s);
+
 // Workgroups of k_stage (stage 2, the larger) that fit on one CU at once (occupancy query).
 template <typename R>
 int stage_blocks_per_cu(int limiter, bool nonideal, bool remap);

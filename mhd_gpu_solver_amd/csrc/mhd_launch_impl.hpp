@@ -150,6 +150,22 @@ int batch_blocks_per_cu(int limiter, bool nonideal)
 }
 
 template <typename R>
+hipError_t launch_max_div_b(const R* U, const Geom& g, double inv_2dx, double inv_2dy,
+                            unsigned long long* out, hipStream_t s)
+{
+    const hipError_t e = hipMemsetAsync(out, 0, sizeof(unsigned long long), s);
+    if (e != hipSuccess) {
+        return e;
+    }
+    const long long n = static_cast<long long>(g.nx) * g.ny;
+    const int blocks = static_cast<int>(std::max<long long>(
+        1, std::min<long long>(kTotalsBlocks, (n + kTotalsBlock - 1) / kTotalsBlock)));
+    hipLaunchKernelGGL((k_max_div_b<R>), dim3(blocks), dim3(kTotalsBlock), 0, s, U, g, inv_2dx,
+                       inv_2dy, out);
+    return hipGetLastError();
+}
+
+template <typename R>
 hipError_t launch_totals(const R* U, const Geom& g, DD* partial, double* out, hipStream_t s)
 {
     hipLaunchKernelGGL((k_totals_partial<R>), dim3(kTotalsBlocks), dim3(kTotalsBlock), 0, s, U, g,
@@ -188,6 +204,8 @@ int stage_blocks_per_cu(int limiter, bool nonideal, bool remap)
                                               hipStream_t);                                       \
     template int batch_blocks_per_cu<R>(int, bool);                                              \
     template int stage_blocks_per_cu<R>(int, bool, bool);                                        \
-    template hipError_t launch_totals<R>(const R*, const Geom&, DD*, double*, hipStream_t);
+    template hipError_t launch_totals<R>(const R*, const Geom&, DD*, double*, hipStream_t);     \
+    template hipError_t launch_max_div_b<R>(const R*, const Geom&, double, double,              \
+                                            unsigned long long*, hipStream_t);
 
 }  // namespace mhdhip

@@ -752,6 +752,33 @@ struct mhd_hip_solver {
         check(hipStreamSynchronize(stream), "hipStreamSynchronize");
     }
 
+    double max_div_b()
+    {
+        hipSetDevice(device);
+        if (!totals_out) {
+            check(hipMalloc(reinterpret_cast<void**>(&totals_out), 9 * sizeof(double)),
+                  "hipMalloc(totals)");
+        }
+        // include/grid/grid.hpp:24-25 (dx, dy), inititial_condition.hpp:276-277 (0.5 / d)
+        const double dx = (grid.x_max - grid.x_min) / static_cast<double>(grid.nx);
+        const double dy = (grid.y_max - grid.y_min) / static_cast<double>(grid.ny);
+        auto* out = reinterpret_cast<unsigned long long*>(totals_out);
+        if (precision == MHD_HIP_FP64) {
+            check(mhdhip::launch_max_div_b<double>(static_cast<double*>(U), geom, 0.5 / dx,
+                                                   0.5 / dy, out, stream),
+                  "max_div_b");
+        } else {
+            check(mhdhip::launch_max_div_b<float>(static_cast<float*>(U), geom, 0.5 / dx,
+                                                  0.5 / dy, out, stream),
+                  "max_div_b");
+        }
+        double v = 0.0;
+        check(hipMemcpyAsync(&v, totals_out, sizeof(double), hipMemcpyDeviceToHost, stream),
+              "hipMemcpyAsync(max_div_b)");
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        return v;
+    }
+
     void set_dt_log(int cap)
     {
         if (cap < 0) {
@@ -944,6 +971,14 @@ int mhd_hip_totals(mhd_hip_solver* s, double sums[9])
     return guarded([&] {
         if (!s || !sums) throw InvalidArgument("null argument");
         s->totals(sums);
+    });
+}
+
+int mhd_hip_max_div_b(mhd_hip_solver* s, double* value)
+{
+    return guarded([&] {
+        if (!s || !value) throw InvalidArgument("null argument");
+        *value = s->max_div_b();
     });
 }
 

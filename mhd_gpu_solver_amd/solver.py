@@ -216,6 +216,13 @@ class DeviceSolver:
         _lib.check(self._l.mhd_hip_totals(self.handle, out))
         return np.array(out[:])
 
+    def max_div_b(self) -> float:
+        """max |div B| of the device state (central differences, ghosts included), on the
+        device; equals MHDSolver.max_div_b of the same state."""
+        v = C.c_double()
+        _lib.check(self._l.mhd_hip_max_div_b(self.handle, C.byref(v)))
+        return v.value
+
     def set_seg_rows(self, rows: int):
         """Rows marched per wavefront (0 = autotuned); results do not depend on it."""
         _lib.check(self._l.mhd_hip_set_seg_rows(self.handle, int(rows)))

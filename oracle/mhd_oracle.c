@@ -715,6 +715,26 @@ void orc_rhs(const orc_grid* g, const double* U, double* rhs, const orc_params* 
 }
 
 /* ---- CFL: include/mhd/cfl.hpp:28-92 ------------------------------------------------------- */
+/* include/initial_condition/inititial_condition.hpp:273-287 */
+double orc_max_div_b(const orc_grid* g, const double* U)
+{
+    const geom q = make_geom(g);
+    const double inv_2dx = 0.5 / q.dx;
+    const double inv_2dy = 0.5 / q.dy;
+    const double* bx = U + 5 * q.n;
+    const double* by = U + 6 * q.n;
+    double max_abs = 0.0;
+    for (int j = q.ng; j < q.ng + q.ny; ++j) {
+        for (int i = q.ng; i < q.ng + q.nx; ++i) {
+            const size_t c = (size_t)j * q.nxt + i;
+            const double dbx = (bx[c + 1] - bx[c - 1]) * inv_2dx;
+            const double dby = (by[c + q.nxt] - by[c - q.nxt]) * inv_2dy;
+            max_abs = vmax(max_abs, fabs(dbx + dby));
+        }
+    }
+    return max_abs;
+}
+
 double orc_cfl(const orc_grid* g, const double* U, const orc_params* p, const uint8_t* mask,
                double* c_h_out)
 {

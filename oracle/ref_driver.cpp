@@ -262,6 +262,15 @@ double ref_cfl(int nx, int ny, int ng, double x0, double x1, double y0, double y
     return r.dt;
 }
 
+double ref_max_div_b(int nx, int ny, int ng, double x0, double x1, double y0, double y1,
+                     const double* Uin)
+{
+    const Grid2D grid = make_grid(nx, ny, ng, x0, x1, y0, y1);
+    StateField U(grid.get_size_x(), grid.get_size_y());
+    load_state(U, Uin);
+    return max_abs_div_b(U, grid);
+}
+
 void ref_face_flux(int limiter, const double* Wmm, const double* Wm, const double* W0,
                    const double* Wp, double gamma, double inv_gm1, double c_h, double* F)
 {

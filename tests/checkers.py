@@ -112,6 +112,8 @@ class Oracle:
         lib.orc_face_flux.restype = None
         lib.orc_face_flux.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_double,
                                       C.c_double, _dp]
+        lib.orc_max_div_b.restype = C.c_double
+        lib.orc_max_div_b.argtypes = [C.POINTER(Grid), _dp]
         lib.orc_cell_max_speed.restype = C.c_double
         lib.orc_cell_max_speed.argtypes = [_dp, C.c_double]
         lib.orc_fast_speed.restype = C.c_double
@@ -165,6 +167,9 @@ class Oracle:
         self.lib.orc_rhs(C.byref(g), np.ascontiguousarray(U), R, C.byref(params), c_h, mp)
         return R
 
+    def max_div_b(self, g, U):
+        return self.lib.orc_max_div_b(C.byref(g), np.ascontiguousarray(U))
+
     def cfl(self, g, U, params, mask=None):
         ch = C.c_double()
         m, mp = _mask_ptr(mask)
@@ -206,6 +211,9 @@ class Reference:
         lib.ref_face_flux.restype = None
         lib.ref_face_flux.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_double, C.c_double,
                                       C.c_double, _dp]
+        lib.ref_max_div_b.restype = C.c_double
+        lib.ref_max_div_b.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                      C.c_double, C.c_double, _dp]
         lib.ref_cell_max_speed.restype = C.c_double
         lib.ref_cell_max_speed.argtypes = [_dp, C.c_double]
         lib.ref_nonideal_e.restype = None
@@ -261,6 +269,9 @@ class Reference:
         m, mp = _mask_ptr(mask)
         self.lib.ref_rhs(*self._g(g), np.ascontiguousarray(U), R, C.byref(params), c_h, mp)
         return R
+
+    def max_div_b(self, g, U):
+        return self.lib.ref_max_div_b(*self._g(g), np.ascontiguousarray(U))
 
     def cfl(self, g, U, params, mask=None):
         ch = C.c_double()

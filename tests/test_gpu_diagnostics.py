@@ -57,3 +57,37 @@ def test_batch_totals_match_single_run(hip):
     i = eng.add_steps(U0, 16, gamma=spec.gamma, bc=spec.bc)
     eng.advance()
     assert_dd_sum(eng.totals(i), eng.download(i))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_device_max_div_b_equals_oracle(hip, oracle, precision):
+    """Device max |div B| after a solve equals the oracle's on the downloaded state, bit for
+    bit (fp64 arithmetic from the stored values in both precisions); on the OT 32^2 golden
+    case it also reproduces the golden's recorded value."""
+    from checkers import MC, make_grid, make_params
+    s = hip.MHDSolver(32, 32)
+    s.set_backend("hip", precision)
+    s.set_limiter_mc()
+    s.initialize("orszag_tang")
+    s.run(0.05)
+    dev = s._dev()
+    got = dev.max_div_b()
+    assert got == oracle.max_div_b(make_grid(32, 32), s.state)
+    assert got == s.max_div_b()
+    if precision == "fp64":
+        assert abs(got - 1.713604) <= 5e-7   # tests/goldens/orszag_tang_32.golden: max_div_b
+
+
+def test_batch_max_div_b_equals_oracle(hip, oracle):
+    from checkers import make_grid
+    from mhd_gpu_solver_amd import workloads
+    eng = hip.BatchEngine("fp32", hbm_budget=1 << 28)
+    specs = [workloads.Spec("rotor", 48, 5.0 / 3.0, 1.0, (2, 2), 0.01),
+             workloads.Spec("kh_tanh", 40, 5.0 / 3.0, 1.0, (0, 0), 0.01, seed=3)]
+    ids = [eng.add(sp.state(), sp.t_end, gamma=sp.gamma, bc=sp.bc) for sp in specs]
+    eng.advance()
+    vals = []
+    for i, sp in zip(ids, specs):
+        vals.append(eng.max_div_b(i))
+        assert vals[-1] == oracle.max_div_b(make_grid(sp.n, sp.n), eng.download(i))
+    assert max(vals) > 0.0

@@ -18,8 +18,8 @@ import pathlib
 import numpy as np
 import pytest
 
-from checkers import (FIELDS, MC, MINMOD, OUTFLOW, PERIODIC, empty_state, interior, make_grid,
-                      make_params, read_golden, read_ref_scalars)
+from checkers import (FIELDS, MC, MINMOD, OUTFLOW, PERIODIC, empty_state, exact_reference,
+                      interior, make_grid, make_params, read_golden, read_ref_scalars)
 
 GOLD = pathlib.Path(__file__).parent / "golden"
 
@@ -40,6 +40,22 @@ def test_orszag_tang_32_golden(oracle):
         got = interior(U)[FIELDS.index(name)].ravel()
         assert rel_l2(got, ref) <= 1e-8, name
         assert np.abs(got - ref).max() <= 1e-7, name
+
+
+def test_max_div_b_golden_and_reference(oracle):
+    """max |div B| after the OT 32^2 golden solve: the golden's recorded value (6 decimals),
+    and bit-equal to the reference's max_abs_div_b when the reference build is present."""
+    g = make_grid(32, 32)
+    U, _ = oracle.solve(g, oracle.init_ic("orszag_tang", g), make_params(t_end=0.05, limiter=MC))
+    meta, _ = read_golden(GOLD / "orszag_tang_32.golden")
+    got = oracle.max_div_b(g, U)
+    assert abs(got - float(meta["max_div_b"])) <= 5e-7
+    U0 = oracle.init_ic("orszag_tang", g)
+    # Bx depends on y only and By on x only: every central difference is exactly zero
+    assert oracle.max_div_b(g, U0) == 0.0
+    ref = exact_reference()
+    if ref is not None:
+        assert ref.max_div_b(g, U) == got
 
 
 def test_ohmic_sine_bz_32_golden(oracle):
