@@ -4,6 +4,7 @@ Hand-written HIP kernels (gfx950) for the SSP-RK2 MUSCL + HLL + GLM (+ Ohmic/Hal
 step, behind a C ABI (include/mhd_hip.h) and a Python mirror of the reference's ``mhd_solver``
 module. Build the library with ``python -m mhd_gpu_solver_amd.build``.
 """
+from . import state_io  # noqa: F401
 from .batch import BatchEngine, CapacityFull, instance_bytes  # noqa: F401
 from .solver import (  # noqa: F401
     FIELDS,
@@ -25,5 +26,5 @@ from .solver import (  # noqa: F401
 __all__ = [
     "BatchEngine", "CapacityFull", "instance_bytes", "FIELDS", "BoundaryConditionType", "DeviceSolver", "MHDSolver", "NonIdealConfig",
     "SlopeLimiter", "SolveParams", "SolveResult", "available_backends", "cuda_available",
-    "hip_available", "init_ic", "metal_available", "wall_plan",
+    "hip_available", "init_ic", "metal_available", "wall_plan", "state_io",
 ]

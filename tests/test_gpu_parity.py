@@ -5,10 +5,14 @@ fp64 must reproduce the oracle's dt sequence bit for bit and every conserved fie
 1e-12 relative L1 (in practice bit-exact); fp32-bulk must stay within 1e-4 relative L1 per field.
 Each case starts the device and the oracle from the same host state.
 """
+import pathlib
+
 import numpy as np
 import pytest
 
 from checkers import MC, MINMOD, OUTFLOW, PERIODIC, FIELDS, make_grid, make_params
+
+GOLD = pathlib.Path(__file__).parent / "golden"
 
 pytestmark = pytest.mark.gpu
 
@@ -151,3 +155,59 @@ def test_results_independent_of_tiling(hip, oracle, rows):
     Uo, info = oracle.solve(grid_of(s), U0, oracle_params(s, 0.01))
     assert r.steps == info["steps"]
     assert np.array_equal(s.state, Uo)
+
+
+def _compare_golden(got, ref, rtol, atol):
+    """tests/test_regression_golden.cpp:125-137: per field rel L2 <= rtol, Linf <= max(atol,
+    10 rtol); steps recorded identically."""
+    assert got.meta["steps"] == ref.meta["steps"]
+    for name, b in ref.fields.items():
+        a = got.fields[name]
+        assert a.size == b.size, name
+        rl2 = np.sqrt(np.sum((a - b) ** 2)) / max(np.sqrt(np.sum(b ** 2)), 1e-300)
+        assert rl2 <= rtol, (name, rl2)
+        assert np.abs(a - b).max() <= max(atol, rtol * 10.0), name
+
+
+def test_orszag_tang_32_golden_regression_on_device(hip):
+    """The reference's OrszagTang32MatchesGolden recipe (test_regression_golden.cpp:50-79,
+    140-156) on the hip backend, fp64, snapshot written through the package's golden I/O."""
+    from mhd_gpu_solver_amd import state_io
+    s = hip.MHDSolver(32, 32)
+    s.set_backend("hip", "fp64")
+    s.set_cfl(0.4)
+    s.set_gamma(5.0 / 3.0)
+    s.set_glm_alpha(0.1)
+    s.set_limiter_mc()
+    s.set_periodic_bc()
+    s.set_ideal()
+    s.initialize("orszag_tang")
+    s.run(0.05)
+    got = state_io.snapshot(s, "orszag_tang_32", ["rho", "bx", "by", "energy"],
+                            {"t_end": "0.05", "cfl": "0.4", "gamma": "1.6666666666666667",
+                             "glm_alpha": "0.1", "limiter": "MC"})
+    ref = state_io.read_golden(GOLD / "orszag_tang_32.golden")
+    _compare_golden(got, ref, 1e-8, 1e-10)
+    assert got.meta == ref.meta
+
+
+def test_ohmic_sine_bz_32_golden_regression_on_device(hip):
+    """OhmicSineBz32MatchesGolden (test_regression_golden.cpp:82-125, 158-171) on hip fp64."""
+    from mhd_gpu_solver_amd import state_io
+    s = hip.MHDSolver(32, 32)
+    s.set_backend("hip", "fp64")
+    s.set_cfl(0.3)
+    s.set_gamma(5.0 / 3.0)
+    s.set_glm_alpha(0.1)
+    s.set_limiter_minmod()
+    s.set_periodic_bc()
+    s.enable_ohmic(0.1)
+    x = (np.arange(32) + 0.5) / 32.0
+    one = np.ones((32, 32))
+    zero = np.zeros((32, 32))
+    s.set_primitives(one, zero, zero, zero, one, zero, zero,
+                     np.broadcast_to(np.sin(2.0 * np.pi * x), (32, 32)))
+    s.run(0.1)
+    got = state_io.snapshot(s, "ohmic_sine_bz_32", ["bz", "energy"])
+    ref = state_io.read_golden(GOLD / "ohmic_sine_bz_32.golden")
+    _compare_golden(got, ref, 5e-8, 1e-9)
