@@ -877,7 +877,13 @@ int mhd_hip_device_name(const mhd_hip_solver* s, char* buf, size_t len)
         }
         hipDeviceProp_t prop;
         check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
-        std::snprintf(buf, len, "HIP: %s (%s)", prop.name, prop.gcnArchName);
+        // the marketing name can be empty (some drivers and containers): fall back to the
+        // device index so the string still names one device
+        if (prop.name[0] != '\0') {
+            std::snprintf(buf, len, "HIP: %s (%s)", prop.name, prop.gcnArchName);
+        } else {
+            std::snprintf(buf, len, "HIP: device %d (%s)", dev, prop.gcnArchName);
+        }
     });
 }
 

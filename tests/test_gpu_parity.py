@@ -211,3 +211,13 @@ def test_ohmic_sine_bz_32_golden_regression_on_device(hip):
     got = state_io.snapshot(s, "ohmic_sine_bz_32", ["bz", "energy"])
     ref = state_io.read_golden(GOLD / "ohmic_sine_bz_32.golden")
     _compare_golden(got, ref, 5e-8, 1e-9)
+
+
+def test_device_name_and_precision(hip):
+    """DeviceSolver::device_name / precision (device_solver.hpp:57-62) through the C ABI."""
+    for precision in ("fp64", "fp32"):
+        s = hip.MHDSolver(16, 16)
+        s.set_backend("hip", precision)
+        name = s.backend_device()
+        assert name.startswith("HIP: ") and "gfx950" in name and "  " not in name, name
+        assert s.precision == precision and s._dev().precision() == precision
