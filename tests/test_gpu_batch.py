@@ -117,3 +117,29 @@ def test_batch_tiling_independent(hip, oracle, rows):
     Uo, info = oracle_run(oracle, s, U0, MC)
     assert eng.result(i)[0].steps == info["steps"]
     assert np.array_equal(eng.download(i), Uo)
+
+
+@pytest.mark.parametrize("limiter", [MINMOD, MC])
+def test_batch_nonideal_rectangular_fp64_matches_oracle(hip, oracle, limiter):
+    """All three non-ideal terms (the batch's NI kernels), rectangular grids, offset domains and
+    mixed boundaries: every instance equals the oracle bit for bit (state, steps, t, c_h)."""
+    ni = hip.NonIdealConfig(ohmic=True, hall=True, ambipolar=True, eta_ohm=1e-3,
+                            eta_hall=1e-2, eta_ambipolar=1e-3)
+    eng = hip.BatchEngine("fp64", limiter=hip.SlopeLimiter(limiter), nonideal=ni,
+                          hbm_budget=1 << 28)
+    cases = [("rotor", 70, 45, (OUTFLOW, OUTFLOW), (-0.5, 0.5, -0.25, 0.5)),
+             ("orszag_tang", 33, 61, (PERIODIC, PERIODIC), (0.0, 1.0, 0.0, 2.0)),
+             ("rotor", 128, 20, (PERIODIC, OUTFLOW), (0.0, 2.0, 0.0, 1.0))]
+    runs = []
+    for name, nx, ny, bc, (x0, x1, y0, y1) in cases:
+        g = make_grid(nx, ny, x0, x1, y0, y1)
+        U0 = oracle.init_ic(name, g)
+        i = eng.add(U0, 0.004, gamma=5.0 / 3.0, bc=bc, x_min=x0, x_max=x1, y_min=y0, y_max=y1)
+        runs.append((i, g, U0, bc))
+    eng.advance()
+    for i, g, U0, bc in runs:
+        r, fin = eng.result(i)
+        Uo, info = oracle.solve(g, U0, make_params(t_end=0.004, limiter=limiter, eta_ohm=1e-3,
+                                                   eta_hall=1e-2, eta_ambipolar=1e-3), bc=bc)
+        assert fin and r.steps == info["steps"] and r.t == info["t"] and r.c_h == info["c_h"]
+        assert np.array_equal(eng.download(i), Uo), (g.nx, g.ny, bc)
