@@ -270,12 +270,12 @@ def copy_probe(nbytes, repeats=3):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=48, help="timed steps (multiple of 8)")
-    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--steps", type=int, default=200, help="timed steps (multiple of 8)")
+    ap.add_argument("--warmup", type=int, default=16)
     ap.add_argument("--instances", type=int, default=512)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--single-n", type=int, default=4096)
-    ap.add_argument("--single-steps", type=int, default=48)
+    ap.add_argument("--single-steps", type=int, default=100)
     ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
