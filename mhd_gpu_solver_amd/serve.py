@@ -37,6 +37,7 @@ class Completed:
     steps: int
     t: float
     state: np.ndarray | None = None
+    error: str | None = None  # the reference's solve() error for this request, if any
 
     @property
     def latency(self):
@@ -55,13 +56,14 @@ class ServeReport:
     replays: int
 
     def latencies_ms(self):
-        return np.array([c.latency for c in self.completed]) * 1e3
+        return np.array([c.latency for c in self.completed if c.error is None]) * 1e3
 
     def summary(self) -> dict:
         lat = self.latencies_ms()
         cu = sum(c.cell_updates for c in self.completed)
         return {
             "requests": len(self.completed),
+            "failed": sum(1 for c in self.completed if c.error is not None),
             "p50_ms": round(float(np.percentile(lat, 50)), 2) if len(lat) else None,
             "p99_ms": round(float(np.percentile(lat, 99)), 2) if len(lat) else None,
             "mean_ms": round(float(lat.mean()), 2) if len(lat) else None,
@@ -116,7 +118,8 @@ class Server:
                     raise ValueError(f"request {k} ({req.n}^2) exceeds the whole HBM budget")
                 try:
                     iid = self.engine.add(req.state(), req.t_end, cfl=self.cfl, gamma=req.gamma,
-                                          glm_alpha=self.glm_alpha, bc=req.bc)
+                                          glm_alpha=self.glm_alpha, bc=req.bc,
+                                          max_steps=req.max_steps)
                 except CapacityFull:
                     break
                 queue.popleft()
@@ -129,11 +132,17 @@ class Server:
             self.engine.advance(max_replays=1)
             replays += 1
             for iid in self.engine.poll_finished():
-                res, _ = self.engine.result(iid)
                 k, adm = active.pop(iid)
-                state = self.engine.download(iid) if keep_states else None
-                self.engine.remove(iid)
                 req = requests[k]
+                try:
+                    res, _ = self.engine.result(iid)
+                    steps, t, err = res.steps, res.t, None
+                except (ValueError, RuntimeError) as e:
+                    # the request's own solve failed (dt <= 0, max_steps before t_end): report
+                    # it with the request and keep serving the others
+                    steps, t, err = 0, float("nan"), str(e)
+                state = self.engine.download(iid) if keep_states and err is None else None
+                self.engine.remove(iid)
                 done.append(Completed(k, req.n, req.ic, req.arrival if realtime else 0.0, adm,
-                                      clock() - t0, res.steps, res.t, state))
+                                      clock() - t0, steps, t, state, err))
         return ServeReport(done, clock() - t0, max_conc, replays)

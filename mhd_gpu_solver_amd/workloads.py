@@ -18,7 +18,8 @@ PERIODIC, OUTFLOW = 0, 2
 
 @dataclasses.dataclass
 class Spec:
-    """One solve: initial condition, grid, gamma, field scale, boundaries, end time."""
+    """One solve: initial condition, grid, gamma, field scale, boundaries, end time (and the
+    step limit past which the solve fails, as the reference's solve() does)."""
     ic: str
     n: int
     gamma: float
@@ -27,6 +28,7 @@ class Spec:
     t_end: float = math.inf
     seed: int = 0
     arrival: float = 0.0
+    max_steps: int = 1_000_000
 
     def state(self):
         return init_ic(self.ic, self.n, self.n, gamma=self.gamma, b0=self.b0, seed=self.seed)

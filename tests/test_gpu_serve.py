@@ -127,3 +127,23 @@ def test_served_fp64_matches_oracle(hip, oracle):
                                 bc=req.bc)
         assert c.steps == info["steps"]
         assert np.array_equal(c.state, Uo)
+
+
+def test_failed_request_is_reported_and_others_complete(hip):
+    """A request that hits the reference's solve() error (max_steps before t_end, forced from
+    the host) is reported with its error; the other requests are served normally."""
+    from mhd_gpu_solver_amd.serve import Server
+    reqs = small_stream(seed=12, count=3)
+    reqs[1].t_end = 10.0
+    reqs[1].max_steps = 3
+    srv = Server("fp32", hbm_budget=1 << 28, max_instances=8, policy="srpt")
+    rep = srv.run(reqs, realtime=False, keep_states=True)
+    by = {c.index: c for c in rep.completed}
+    assert len(by) == 3
+    assert by[1].error is not None and "max_steps" in by[1].error and by[1].state is None
+    for k in (0, 2):
+        ref, r = alone(hip, reqs[k])
+        assert by[k].error is None and by[k].steps == r.steps
+        assert np.array_equal(by[k].state, ref)
+    s = rep.summary()
+    assert s["failed"] == 1 and s["requests"] == 3
