@@ -132,3 +132,23 @@ def test_server_rejects_unknown_policy():
     from mhd_gpu_solver_amd.serve import Server
     with pytest.raises(ValueError):
         Server(policy="lifo")
+
+
+def test_execution_settings_keep_reference_semantics(monkeypatch):
+    """set_num_threads / num_threads / set_simd_enabled / simd_* (python/mhd_pybind.cpp:169-180,
+    include/mhd/parallel.hpp:201-256): accepted with the reference's meaning; 0 = default."""
+    import os
+    monkeypatch.setenv("MHD_NUM_THREADS", "5")
+    m.set_num_threads(0)
+    assert m.num_threads() == 5
+    m.set_num_threads(3)
+    assert m.num_threads() == 3
+    m.set_num_threads(-2)
+    assert m.num_threads() == 5
+    monkeypatch.setenv("MHD_NUM_THREADS", "junk")
+    assert m.num_threads() == (os.cpu_count() or 1)
+    m.set_simd_enabled(False)
+    assert m.simd_enabled() is False
+    m.set_simd_enabled(True)
+    assert m.simd_enabled() is True
+    assert m.simd_backend() == "scalar" and m.simd_width() == 1
