@@ -1,7 +1,7 @@
-"""The C++ adapter that INTEGRATION.md asks a maintainer to add (HipSolver : DeviceSolver over the
-C ABI) compiles against the reference's real DeviceSolver interface and this backend's
-include/mhd_hip.h. Needs the reference sources, which exist only where they were unpacked; the
-test skips elsewhere."""
+"""The C++ adapter documented for maintainers of the reference (HipSolver : DeviceSolver over the
+C ABI, section 2 of the integration notes at the repository root) compiles against the
+reference's real DeviceSolver interface and this backend's include/mhd_hip.h. Needs the
+reference sources, which exist only where they were unpacked; the test skips elsewhere."""
 import pathlib
 import re
 import shutil
@@ -12,7 +12,7 @@ import pytest
 ROOT = pathlib.Path(__file__).resolve().parent.parent
 REF_INCLUDE = pathlib.Path("/root/reference/include")
 
-# the two declarations section 1 of INTEGRATION.md adds to device_solver.hpp
+# the two declarations section 1 of the integration notes adds to device_solver.hpp
 PRELUDE = """#include <memory>
 #include "mhd/gpu/device_solver.hpp"
 namespace mhd {
