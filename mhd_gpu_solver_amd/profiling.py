@@ -1,11 +1,13 @@
 """Summaries of rocprofv3 CSV output for the hip backend's kernels.
 
-    python -m mhd_gpu_solver_amd.profiling trace  <kernel_trace.csv>
+    python -m mhd_gpu_solver_amd.profiling trace  <kernel_trace.csv> [last N]
     python -m mhd_gpu_solver_amd.profiling traffic <fetch counter_collection.csv> \
         <write counter_collection.csv> <copy bytes> <interior cells per launch> [kernel filter]
 
 ``trace`` prints per-kernel launch statistics. Launches shorter than ``NOOP_US`` are the
 early-exit steps at the end of a fixed-size batch and are counted apart from the active ones.
+With ``last N`` it adds the mean of each kernel's last N active launches: the timed steps of a
+bench.py leg come last, after its autotuning and warm-up launches.
 ``traffic`` converts FETCH_SIZE / WRITE_SIZE (KiB per dispatch) into bytes per launch, scaled
 by what the same counter reads for a plain device copy of known size in the same run.
 """
@@ -23,23 +25,30 @@ def _short(name: str) -> str:
     return name.split("(")[0].replace("void ", "").replace("mhdhip::", "")
 
 
-def kernel_table(trace_csv: str) -> str:
-    rows = list(csv.DictReader(open(trace_csv)))
+def kernel_table(trace_csv: str, last: int = 0) -> str:
+    rows = sorted(csv.DictReader(open(trace_csv)), key=lambda r: int(r["Start_Timestamp"]))
     durs = collections.defaultdict(list)
     for r in rows:
         d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
         durs[_short(r["Kernel_Name"])].append(d)
-    out = ["| kernel | launches | active | mean active (us) | min (us) | max (us) | no-op mean (us) |",
-           "|---|---|---|---|---|---|---|"]
+    tail = f" mean of last {last} (us) |" if last else ""
+    out = ["| kernel | launches | active | mean active (us) | min (us) | max (us) | no-op mean (us) |"
+           + tail, "|---|---|---|---|---|---|---|" + ("---|" if last else "")]
     for k, v in sorted(durs.items(), key=lambda kv: -sum(kv[1])):
         act = [x for x in v if x > NOOP_US]
         noop = [x for x in v if x <= NOOP_US]
         if act:
-            out.append(f"| {k} | {len(v)} | {len(act)} | {sum(act) / len(act):.1f} | "
-                       f"{min(act):.1f} | {max(act):.1f} | "
-                       f"{(sum(noop) / len(noop)) if noop else 0:.2f} |")
+            line = (f"| {k} | {len(v)} | {len(act)} | {sum(act) / len(act):.1f} | "
+                    f"{min(act):.1f} | {max(act):.1f} | "
+                    f"{(sum(noop) / len(noop)) if noop else 0:.2f} |")
+            if last:
+                sel = act[-last:]
+                line += f" {sum(sel) / len(sel):.1f} |"
         else:
-            out.append(f"| {k} | {len(v)} | 0 | - | - | - | {sum(noop) / len(noop):.2f} |")
+            line = f"| {k} | {len(v)} | 0 | - | - | - | {sum(noop) / len(noop):.2f} |"
+            if last:
+                line += " - |"
+        out.append(line)
     return "\n".join(out)
 
 
@@ -88,7 +97,8 @@ def traffic(fetch_csv: str, write_csv: str, copy_bytes: int, cells: int,
 
 def main(argv):
     if len(argv) >= 2 and argv[0] == "trace":
-        print(kernel_table(argv[1]))
+        last = int(argv[3]) if len(argv) >= 4 and argv[2] == "last" else 0
+        print(kernel_table(argv[1], last))
     elif len(argv) >= 5 and argv[0] == "traffic":
         filt = argv[5] if len(argv) >= 6 else "stage"
         print(json.dumps(traffic(argv[1], argv[2], int(argv[3]), int(argv[4]), filt), indent=1))
