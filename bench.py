@@ -240,9 +240,19 @@ def cpu_leg(specs, states, steps, budget_s=12.0):
         count += 1
         if secs > budget_s:
             break
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {
         "value": round(done_cells / secs / 1e6, 2), "unit": "Mcell-updates/s", "cores": threads,
         "kind": "reference",
+        "host": {"cpu_model": model, "nproc": os.cpu_count(),
+                 "isa": ref.lib.ref_simd_backend().decode()},
         "sample": f"first {count} of {len(specs)} ensemble instances, {steps} SSP-RK2 steps each, "
                   f"fp64, {secs:.1f} s, reference CPU solver ({path.name}, "
                   f"{ref.lib.ref_simd_backend().decode()} SIMD, {threads} threads)",
