@@ -10,10 +10,11 @@
 //
 // Precision policy (template parameter R):
 //   double: everything in fp64, in the reference's operation order.
-//   float : bulk arithmetic in fp32. Pressure recovery from total energy is evaluated in fp64
-//           (MixedOps<float>), because it subtracts nearly equal quantities at low plasma beta
-//           or high Mach number; the fast-speed discriminant uses a cancellation-free form and
-//           the HLL flux clamped wave speeds (same values up to rounding, fewer instructions).
+//   float : bulk arithmetic in fp32. Pressure recovery from total energy and the fast-speed
+//           discriminant are evaluated in fp64 (MixedOps<float>), because both subtract nearly
+//           equal quantities (low plasma beta or high Mach number; strong field along the
+//           normal); the HLL flux uses clamped wave speeds (same values up to rounding, fewer
+//           instructions).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -141,16 +142,6 @@ __device__ __forceinline__ R fast_speed(const R (&W)[kNC], R gamma)
     const R inv_rho = drcp(W[RHO]);
     const R cs2 = gamma * W[PR] * inv_rho;
     const R bn2 = W[D::BN] * W[D::BN];
-    if constexpr (sizeof(R) == 4) {
-        // fp32: (cs2 + va2)^2 - 4 cs2 van2 == (cs2 - va2)^2 + 4 cs2 (va2 - van2), and
-        // va2 - van2 = (bt^2 + bz^2) / rho >= 0, so this form has no cancellation
-        const R bp2 = W[D::BT] * W[D::BT] + W[BZ] * W[BZ];
-        const R va2 = (bn2 + bp2) * inv_rho;
-        const R dlt = cs2 - va2;
-        const R disc = dlt * dlt + R(4) * cs2 * (bp2 * inv_rho);
-        const R cf2 = half * (cs2 + va2 + dsqrt(disc));
-        return dsqrt(vmax(cf2, R(0)));
-    }
     const R b2 = bn2 + W[D::BT] * W[D::BT] + W[BZ] * W[BZ];
     const R va2 = b2 * inv_rho;
     const R van2 = bn2 * inv_rho;
