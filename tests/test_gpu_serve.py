@@ -147,3 +147,24 @@ def test_failed_request_is_reported_and_others_complete(hip):
         assert np.array_equal(by[k].state, ref)
     s = rep.summary()
     assert s["failed"] == 1 and s["requests"] == 3
+
+
+def test_config5_served_fp32_within_l1_of_fp64(hip):
+    """Requests of the serving mix (oblique Alfven, tanh KH, rotor) served in FP32 are within
+    1e-4 relative L1 per field of the same requests solved alone in fp64 (both at t_end)."""
+    from mhd_gpu_solver_amd.serve import Server
+    from mhd_gpu_solver_amd.solver import FIELDS
+    reqs = small_stream(seed=21, count=6)
+    rep = Server("fp32", hbm_budget=1 << 30, max_instances=16).run(reqs, realtime=False,
+                                                                    keep_states=True)
+    worst = {}
+    for c in rep.completed:
+        ref, r = alone(hip, reqs[c.index], "fp64")
+        assert c.t == r.t == reqs[c.index].t_end
+        for k, name in enumerate(FIELDS[:8]):
+            den = np.abs(ref[k, 2:-2, 2:-2]).sum()
+            if den > 0:
+                e = np.abs(c.state[k, 2:-2, 2:-2] - ref[k, 2:-2, 2:-2]).sum() / den
+                worst[name] = max(worst.get(name, 0.0), float(e))
+    print(f"config5 fp32 L1 {worst}")
+    assert max(worst.values()) <= 1e-4, worst
