@@ -149,10 +149,11 @@ int mhd_hip_max_div_b(mhd_hip_solver* s, double* value);
  * stream around the step loop, excluding upload/download). */
 int mhd_hip_last_solve_seconds(const mhd_hip_solver* s, double* seconds);
 
-/* Rows of the grid marched by one wavefront of the stage kernel (the tiling's only free
- * parameter). 0 (default) selects the autotuned value: on the first solve of a shape the
- * stage kernel is timed for candidate lengths and the fastest is kept for the process.
- * Results do not depend on this value. mhd_hip_get_seg_rows reports the value last used. */
+/* Rows of the grid marched by one wavefront of the stage kernels (the tiling's only free
+ * parameter). 0 (default) selects the autotuned value: on the first solve of a shape the pair
+ * of stage kernels of one step is timed for candidate lengths and the fastest is kept for the
+ * process. Results do not depend on this value. mhd_hip_get_seg_rows reports the value last
+ * used. */
 int mhd_hip_set_seg_rows(mhd_hip_solver* s, int rows);
 int mhd_hip_get_seg_rows(mhd_hip_solver* s, int* rows);
 
@@ -211,7 +212,7 @@ typedef struct mhd_hip_batch_stats {
     int instances; /* admitted (running or finished, not yet removed) */
     int active;    /* still stepping (unfinished, not paused) */
     int tiles;     /* wave work items in the current tile list */
-    int seg_rows;  /* rows per wave in use (autotuned unless set) */
+    int seg_rows;  /* rows per wave of the current tile list (load model unless set) */
     int blocks;    /* workgroups of the persistent stage kernel */
     size_t bytes_used, bytes_budget;
     double last_seconds; /* device time of the last mhd_hip_batch_advance */

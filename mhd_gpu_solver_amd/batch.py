@@ -107,6 +107,8 @@ class BatchEngine:
         return v.value
 
     def set_seg_rows(self, rows: int):
+        """Rows per wave: 0 load model (default), -1 timed once on the first advance, > 0
+        fixed. Results do not depend on it."""
         _lib.check(self._l.mhd_hip_batch_set_seg_rows(self.handle, int(rows)))
 
     def set_paused(self, iid: int, paused: bool):

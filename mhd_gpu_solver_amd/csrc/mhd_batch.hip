@@ -623,8 +623,8 @@ struct mhd_hip_batch {
         check(e, "hipGraphInstantiate");
     }
 
-    // Autotuner: times one stage-1 pass (which leaves U untouched) over the current instance
-    // set for candidate row-segment lengths and keeps the fastest.
+    // Timed tuning (rows per wave -1): times one stage-1 pass (which leaves U untouched) over
+    // the current instance set for each candidate length and keeps the fastest.
     template <typename R>
     void autotune()
     {

@@ -39,12 +39,10 @@ template <typename R>
 struct Lit;
 template <>
 struct Lit<double> {
-    static __device__ __forceinline__ double inf() { return __builtin_huge_val(); }
     static constexpr double kDegenerate = 1.0e-14;
 };
 template <>
 struct Lit<float> {
-    static __device__ __forceinline__ float inf() { return __builtin_huge_valf(); }
     static constexpr float kDegenerate = 1.0e-14f;
 };
 
