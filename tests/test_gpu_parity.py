@@ -221,3 +221,44 @@ def test_device_name_and_precision(hip):
         name = s.backend_device()
         assert name.startswith("HIP: ") and "gfx950" in name and "  " not in name, name
         assert s.precision == precision and s._dev().precision() == precision
+
+
+def test_concurrent_solvers_from_threads(hip):
+    """Several solvers may run at once from Python threads (the reference releases the GIL,
+    python/mhd_pybind.cpp:279-290): each result equals the same solve run alone."""
+    import threading
+
+    def make(precision, ic, n):
+        s = hip.MHDSolver(n, n)
+        s.set_backend("hip", precision)
+        s.set_limiter_mc()
+        s.initialize(ic)
+        return s
+
+    cases = [("fp64", "orszag_tang", 96), ("fp32", "rotor", 80), ("fp64", "rotor", 64),
+             ("fp32", "orszag_tang", 112)]
+    alone = []
+    for prec, ic, n in cases:
+        s = make(prec, ic, n)
+        r = s.run(0.02)
+        alone.append((r.steps, s.state.copy()))
+    solvers = [make(*c) for c in cases]
+    out = [None] * len(cases)
+    errors = []
+
+    def work(k):
+        try:
+            r = solvers[k].run(0.02)
+            out[k] = (r.steps, solvers[k].state.copy())
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    threads = [threading.Thread(target=work, args=(k,)) for k in range(len(cases))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=60)
+    assert not errors, errors
+    for (st_a, U_a), res in zip(alone, out):
+        assert res is not None and res[0] == st_a
+        assert np.array_equal(res[1], U_a)
