@@ -361,6 +361,7 @@ def main():
             "batched_fp64": {"value": round(b64["mcups"], 2),
                              "drift_periodic_max": b64["drift"],
                              "ms_per_step": round(b64["ms_per_step"], 4),
+                             "seg_rows": b64["seg_rows"],
                              "roofline": roofline(
                                  b64["kernels"], b64["cells"], b64["ms_per_step"], 8,
                                  "k_batch_stage",

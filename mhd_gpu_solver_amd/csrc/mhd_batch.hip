@@ -499,6 +499,38 @@ struct mhd_hip_batch {
 
     static constexpr int kRowCandidates[] = {8, 12, 16, 20, 24, 28, 32, 40, 48, 64, 96, 128};
 
+    // The tiles of one instance for `rows` rows per wave, in launch order: whole-wave tiles
+    // over its 60-column strips, row segment by row segment; then, when its last strip is at
+    // most kHalfCells columns wide, half-wave tiles for that strip, each pairing a chunk of the
+    // strip's lower half of rows (lanes 0..31) with the same chunk of its upper half
+    // (lanes 32..63). f(strip, r0, r1, r0b, r1b); r0b < 0 marks a whole-wave tile.
+    // Half-wave tiles are made for the fp64 batch only: in the fp32 kernel the second tile
+    // kind costs the registers that keep its stage 2 free of spills.
+    template <typename F>
+    void for_each_tile(const mhd_hip_instance& d, int rows, F&& f) const
+    {
+        const int ng = 2;
+        const int full = d.nx / mhdhip::kStripCells;
+        const int tail = d.nx - full * mhdhip::kStripCells;
+        const bool half =
+            precision == MHD_HIP_FP64 && tail > 0 && tail <= mhdhip::kHalfCells;
+        const int wide = half ? full : (d.nx + mhdhip::kStripCells - 1) / mhdhip::kStripCells;
+        for (int r0 = ng; r0 < ng + d.ny; r0 += rows) {
+            for (int st = 0; st < wide; ++st) {
+                f(st, r0, std::min(r0 + rows, ng + d.ny), -1, 0);
+            }
+        }
+        if (half) {
+            const int h = (d.ny + 1) / 2;  // rows of the lower half (the upper has h or h - 1)
+            for (int a = 0; a < h; a += rows) {
+                const int r0 = ng + a;
+                const int r1 = ng + std::min(a + rows, h);
+                const int r0b = ng + h + a;
+                f(full, r0, r1, r0b, std::min(r0b + (r1 - r0), ng + d.ny));
+            }
+        }
+    }
+
     // Rows per wave for the current load. The persistent kernel hands tile t to wave
     // t mod (waves in the grid); a tile of L rows marches L + 2 rows and loads L + 3, so its
     // cost is taken as 2 L + 5 half-rows. The model replays that assignment for each
@@ -517,13 +549,10 @@ struct mhd_hip_batch {
                 if (!s.used || s.finished || s.paused) {
                     continue;
                 }
-                const int strips = (s.desc.nx + mhdhip::kStripCells - 1) / mhdhip::kStripCells;
-                for (int r0 = 0; r0 < s.desc.ny; r0 += rows) {
-                    const long long c = 2ll * std::min(rows, s.desc.ny - r0) + 5;
-                    for (int st = 0; st < strips; ++st, ++k) {
-                        load[static_cast<size_t>(k % waves)] += c;
-                    }
-                }
+                for_each_tile(s.desc, rows, [&](int, int r0, int r1, int, int) {
+                    load[static_cast<size_t>(k % waves)] += 2ll * (r1 - r0) + 5;
+                    ++k;
+                });
             }
             const long long cost = *std::max_element(load.begin(), load.end());
             if (cost <= best_cost) {
@@ -547,9 +576,7 @@ struct mhd_hip_batch {
         size_t n = 0;
         for (const auto& s : slots) {
             if (s.used && !s.finished && !s.paused) {
-                const int strips = (s.desc.nx + mhdhip::kStripCells - 1) / mhdhip::kStripCells;
-                const int segs = (s.desc.ny + rows - 1) / rows;
-                n += static_cast<size_t>(strips) * segs;
+                for_each_tile(s.desc, rows, [&](int, int, int, int, int) { ++n; });
             }
         }
         ensure_tiles(std::max<size_t>(n, 1));
@@ -571,12 +598,9 @@ struct mhd_hip_batch {
             if (s.finished || s.paused) {
                 continue;
             }
-            const int strips = (s.desc.nx + mhdhip::kStripCells - 1) / mhdhip::kStripCells;
-            for (int r0 = 2; r0 < 2 + s.desc.ny; r0 += rows) {
-                for (int st = 0; st < strips; ++st) {
-                    h_tiles[k++] = BatchTile{si, st, r0, std::min(r0 + rows, 2 + s.desc.ny)};
-                }
-            }
+            for_each_tile(s.desc, rows, [&](int st, int r0, int r1, int r0b, int r1b) {
+                h_tiles[k++] = BatchTile{si, st, r0, r1, r0b, r1b};
+            });
         }
         ntiles = static_cast<int>(k);
         *h_ntiles = ntiles;

@@ -24,7 +24,9 @@ struct BatchInst {
 struct BatchTile {
     int inst;
     int strip;
-    int r0, r1;  // global rows [r0, r1) marched by one wave
+    int r0, r1;   // global rows [r0, r1) marched by the wave (by its lower half when r0b >= 0)
+    int r0b, r1b; // half-wave tile of a narrow strip: the upper half's rows start at r0b and
+                  // are updated below r1b; r0b < 0 for a whole-wave tile
 };
 
 struct BatchTables {
@@ -35,7 +37,8 @@ struct BatchTables {
 };
 
 template <int STAGE, int LIM, int NI, typename R>
-__global__ __launch_bounds__(kBlock) void k_batch_stage(BatchTables tb, int parity)
+__global__ __launch_bounds__(kBlock)
+void k_batch_stage(BatchTables tb, int parity)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = wave_in_block();
@@ -71,9 +74,18 @@ __global__ __launch_bounds__(kBlock) void k_batch_stage(BatchTables tb, int pari
         const R* src = static_cast<const R*>(STAGE == 1 ? in.U : in.Us);
         const R* u0 = static_cast<const R*>(in.U);
         R* dst = static_cast<R*>(STAGE == 1 ? in.Us : in.U);
-        const R local = march_tile<STAGE, LIM, NI, 1, R>(src, u0, dst, nullptr, in.g, in.ph, st,
-                                                         tile.strip, tile.r0, tile.r1, ring[wave],
-                                                         lane);
+        // half-wave tiles (r0b >= 0) come from the fp64 tables only; the fp32 kernel holds
+        // the whole-wave march alone
+        R local;
+        if (sizeof(R) == 4 || tile.r0b < 0) {
+            local = march_tile<STAGE, LIM, NI, 1, 0, 1, R>(src, u0, dst, nullptr, in.g, in.ph, st,
+                                                           tile.strip, tile.r0, tile.r1, 0,
+                                                           tile.r1, ring[wave], lane);
+        } else {
+            local = march_tile<STAGE, LIM, NI, 1, 1, 0, R>(
+                src, u0, dst, nullptr, in.g, in.ph, st, tile.strip, tile.r0, tile.r1,
+                tile.r0b - tile.r0, tile.r1b, ring[wave], lane);
+        }
         if (STAGE == 2) {
             const R m = wave_max(local);
             if (lane == 0) {

@@ -27,6 +27,7 @@ namespace mhdhip {
 
 constexpr int kWave = 64;
 constexpr int kStripCells = kWave - 4;  // cells per wave strip: 2 halo lanes left, 2 right
+constexpr int kHalfCells = kWave / 2 - 4;  // cells per half-wave strip (narrow tail strips)
 constexpr int kWavesPerBlock = 4;
 constexpr int kBlock = kWave * kWavesPerBlock;
 
@@ -65,6 +66,9 @@ __device__ __forceinline__ double dpp_move(double x)
 // 1 and 63 see zero-filled neighbours and are never used; wave-uniform branch decisions on
 // x-faces are taken over these lanes only.
 constexpr unsigned long long kXFaceLanes = ((1ull << 61) - 1ull) << 2;
+// the same for a wave split into two half-wave strips: lanes 2..30 and 34..62
+constexpr unsigned long long kXFaceLanesHalf =
+    (((1ull << 29) - 1ull) << 2) | (((1ull << 29) - 1ull) << 34);
 
 constexpr int kDppWaveShr1 = 0x138;
 constexpr int kDppWaveShl1 = 0x130;
@@ -497,18 +501,32 @@ struct StageBuffers {
 // March rows [r0, r1) of one 60-column strip (see the comment above StageBuffers) and apply the
 // RK stage to its cells. `ring` is this wave's LDS slice. Returns the wave-local maximum CFL
 // signal speed of the updated fluid cells (stage 2; 0 in stage 1).
-template <int STAGE, int LIM, int NI, int REMAP, typename R>
+//
+// HALF: the strip is a narrow one (at most kHalfCells columns, the tail of a grid) and the wave
+// works as two half-wave strips of 32 lanes, each with its own 2 halo lanes on either side.
+// Lanes 0..31 march rows [r0, r1), lanes 32..63 rows [r0 + dB, r1 + dB), of which only those
+// below r1b are updated. Shifts across lanes 31/32 feed only halo lanes, whose faces are never
+// used (the same argument as for the wave's edge lanes).
+//
+// EARLY: stage 2 issues its loads of U for the updated row at the start of the row step
+// (latency overlapped with the faces, nine more live registers); otherwise at the update.
+template <int STAGE, int LIM, int NI, int REMAP, int HALF, int EARLY, typename R>
 __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, R* dst,
                                         const uint8_t* mask, const Geom& g, const Physics& ph,
-                                        const StepState& st, int strip, int r0, int r1,
-                                        R (&ring)[3][kNC][kWave], int lane)
+                                        const StepState& st, int strip, int r0, int r1, int dB,
+                                        int r1b, R (&ring)[3][kNC][kWave], int lane)
 {
     const long long P = g.plane;
 
-    const int gi = g.ng + strip * kStripCells - 2 + lane;
+    const bool upper = HALF && lane >= kWave / 2;
+    const int l = HALF ? (lane & (kWave / 2 - 1)) : lane;  // lane within its strip
+    const int row_off = upper ? dB : 0;                      // this lane's row offset
+    const int gi = g.ng + strip * kStripCells - 2 + l;
     const int gic = min(gi, g.nxt - 1);
     const int col = REMAP ? ghost_source(gic, g.nx, g.ng, g.bc_x) : gic;
-    const bool cell_lane = lane >= 2 && lane < 2 + kStripCells && gi < g.ng + g.nx;
+    const bool cell_lane =
+        l >= 2 && l < 2 + (HALF ? kHalfCells : kStripCells) && gi < g.ng + g.nx;
+    constexpr unsigned long long x_live = HALF ? kXFaceLanesHalf : kXFaceLanes;
 
     const R gamma = static_cast<R>(ph.gamma);
     const R gm1 = static_cast<R>(ph.gm1);
@@ -527,7 +545,8 @@ __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, 
     const R half_dt = static_cast<R>(0.5 * st.dt);
 
     auto load_row = [&](int q, R (&u)[kNC]) {
-        const int qr = REMAP ? ghost_source(q, g.ny, g.ng, g.bc_y) : q;
+        const int ql = q + row_off;
+        const int qr = REMAP ? ghost_source(ql, g.ny, g.ng, g.bc_y) : ql;
         const long long base = static_cast<long long>(qr) * g.nxt + col;
 #pragma unroll
         for (int c = 0; c < kNC; ++c) {
@@ -540,10 +559,12 @@ __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, 
     R Jp[3] = {R(0), R(0), R(0)};
     R local_speed = R(0);
 
+    // ring slot of row q, counted from the segment start so that it is the same on all lanes
+    auto slot_of = [&](int q) { return (q - r0 + 3) % 3; };
     auto ring_put = [&](int q, const R (&u)[kNC]) {
 #pragma unroll
         for (int k = 0; k < kNC; ++k) {
-            ring[q % 3][k][lane] = u[k];
+            ring[slot_of(q)][k][lane] = u[k];
         }
     };
 
@@ -559,11 +580,13 @@ __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, 
         if (c + 2 <= r1 + 1) {
             load_row(c + 2, Uraw);
         }
-        // stage 2 updates row c - 1 from U at the start of the step: issue those loads now so
-        // that their latency overlaps the face computations (column clamped like the row loads)
+        // stage 2 updates row c - 1 from U at the start of the step: with EARLY those loads are
+        // issued now, so that their latency overlaps the face computations (column clamped like
+        // the row loads)
+        constexpr bool early_u0 = STAGE == 2 && EARLY;
         R U0r[kNC];
-        if (STAGE == 2 && c >= r0 + 1) {
-            const long long base = static_cast<long long>(c - 1) * g.nxt + gic;
+        if (early_u0 && c >= r0 + 1) {
+            const long long base = static_cast<long long>(c - 1 + row_off) * g.nxt + gic;
 #pragma unroll
             for (int k = 0; k < kNC; ++k) {
                 U0r[k] = u0[k * P + base];
@@ -622,7 +645,7 @@ __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, 
                     WLx[k] = from_left(Wa[k] + half * d);
                 }
                 R F[kNC];
-                hll_face<0>(WLx, WRx, Wl, Wa, gamma, inv_gm1, c_h, F, kXFaceLanes);
+                hll_face<0>(WLx, WRx, Wl, Wa, gamma, inv_gm1, c_h, F, x_live);
                 if (NI) {
                     // include/mhd/rhs.hpp:74-93
                     R Jf[3], Bf[3], E[3];
@@ -646,16 +669,17 @@ __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, 
                 if (psi_damp != R(0)) {
                     L[PSI] -= psi_damp * Wa[PSI];
                 }
-                if (cell_lane) {
+                const int rl = rr + row_off;  // this lane's row
+                if (cell_lane && (!upper || rl < r1b)) {
                     if (mask != nullptr &&
-                        mask[static_cast<long long>(rr - g.ng) * g.nx + (gi - g.ng)] == 0) {
+                        mask[static_cast<long long>(rl - g.ng) * g.nx + (gi - g.ng)] == 0) {
 #pragma unroll
                         for (int k = 0; k < kNC; ++k) {
                             L[k] = R(0);
                         }
                     }
-                    const long long cell = static_cast<long long>(rr) * g.nxt + gi;
-                    const int slot = rr % 3;
+                    const long long cell = static_cast<long long>(rl) * g.nxt + gi;
+                    const int slot = slot_of(rr);
                     if (STAGE == 1) {
                         // include/mhd/field_ops.hpp:47-53
 #pragma unroll
@@ -667,8 +691,8 @@ __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, 
                         R un[kNC];
 #pragma unroll
                         for (int k = 0; k < kNC; ++k) {
-                            un[k] = half * (U0r[k] + ring[slot][k][lane]) +
-                                    half_dt * L[k];
+                            const R u0k = early_u0 ? U0r[k] : u0[k * P + cell];
+                            un[k] = half * (u0k + ring[slot][k][lane]) + half_dt * L[k];
                         }
 #pragma unroll
                         for (int k = 0; k < kNC; ++k) {
@@ -676,7 +700,7 @@ __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, 
                         }
                         const bool fluid =
                             mask == nullptr ||
-                            mask[static_cast<long long>(rr - g.ng) * g.nx + (gi - g.ng)] != 0;
+                            mask[static_cast<long long>(rl - g.ng) * g.nx + (gi - g.ng)] != 0;
                         if (fluid) {
                             local_speed = vmax(local_speed, cell_max_speed(un, gamma, gm1));
                         }
@@ -748,8 +772,8 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageBuffers b, Geom g, Physic
     // wave-private ring of the last three raw (conserved) rows: the RK update of row c-1 reads
     // its stage input from here instead of global memory
     __shared__ R ring[kWavesPerBlock][3][kNC][kWave];
-    const R local_speed = march_tile<STAGE, LIM, NI, REMAP, R>(src, u0, dst, b.mask, g, ph, st,
-                                                               strip, r0, r1, ring[wave], lane);
+    const R local_speed = march_tile<STAGE, LIM, NI, REMAP, 0, 1, R>(
+        src, u0, dst, b.mask, g, ph, st, strip, r0, r1, 0, r1, ring[wave], lane);
     if (STAGE == 2) {
         const R m = wave_max(local_speed);
         if (lane == 0) {
