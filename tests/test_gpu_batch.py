@@ -143,3 +143,32 @@ def test_batch_nonideal_rectangular_fp64_matches_oracle(hip, oracle, limiter):
                                                    eta_hall=1e-2, eta_ambipolar=1e-3), bc=bc)
         assert fin and r.steps == info["steps"] and r.t == info["t"] and r.c_h == info["c_h"]
         assert np.array_equal(eng.download(i), Uo), (g.nx, g.ny, bc)
+
+
+@pytest.mark.parametrize("rows", [0, 1, 2, 200])
+def test_batch_half_wave_tiles_edges_match_oracle(hip, oracle, rows):
+    """Narrow tail strips (at most 28 columns) march on half-wave tiles in the fp64 batch: the
+    lower and upper halves of a wave take the same chunk of the strip's lower and upper row
+    halves. Edges: tails of 1 and 28 columns, a grid of one tail strip only, 1 to 3 rows (the
+    upper half then updates one row or none), odd and even row counts, every boundary mix and
+    rows per wave from 1 to more than the grid (0: the load model's choice)."""
+    eng = hip.BatchEngine("fp64", limiter=hip.SlopeLimiter.MC, hbm_budget=1 << 28)
+    if rows:
+        eng.set_seg_rows(rows)
+    cases = [("orszag_tang", 61, 1, (PERIODIC, PERIODIC)),
+             ("rotor", 88, 2, (OUTFLOW, OUTFLOW)),
+             ("orszag_tang", 28, 3, (PERIODIC, OUTFLOW)),
+             ("rotor", 121, 9, (OUTFLOW, PERIODIC)),
+             ("orszag_tang", 5, 40, (PERIODIC, PERIODIC))]
+    runs = []
+    for name, nx, ny, bc in cases:
+        g = make_grid(nx, ny, 0.0, 1.0, 0.0, 1.0)
+        U0 = oracle.init_ic(name, g)
+        runs.append((eng.add_steps(U0, 7, bc=bc), g, U0, bc))
+    eng.advance()
+    for i, g, U0, bc in runs:
+        r, fin = eng.result(i)
+        Uo, info = oracle.solve(g, U0, make_params(t_end=1e300, limiter=MC, max_steps=7), bc=bc,
+                                allow_max_steps=True)
+        assert fin and r.steps == 7 and r.t == info["t"] and r.c_h == info["c_h"]
+        assert np.array_equal(eng.download(i), Uo), (g.nx, g.ny, bc, rows)
