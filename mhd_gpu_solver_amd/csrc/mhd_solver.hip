@@ -236,7 +236,7 @@ struct mhd_hip_solver {
         if (mask_dev == nullptr) {
             check(hipMalloc(reinterpret_cast<void**>(&mask_dev), need), "hipMalloc(mask)");
         }
-        check(hipMemcpy(mask_dev, mask, need, hipMemcpyHostToDevice), "hipMemcpy(mask)");
+        copy_sync(mask_dev, mask, need, hipMemcpyHostToDevice, "hipMemcpy(mask)");
         std::vector<long long> plan;
         plan_count = static_cast<int>(mhdhip::wall_plan(grid, mask, plan));
         if (plan_dev) {
@@ -246,10 +246,18 @@ struct mhd_hip_solver {
         if (plan_count > 0) {
             check(hipMalloc(reinterpret_cast<void**>(&plan_dev), plan.size() * sizeof(long long)),
                   "hipMalloc(plan)");
-            check(hipMemcpy(plan_dev, plan.data(), plan.size() * sizeof(long long),
-                            hipMemcpyHostToDevice),
-                  "hipMemcpy(plan)");
+            copy_sync(plan_dev, plan.data(), plan.size() * sizeof(long long),
+                      hipMemcpyHostToDevice, "hipMemcpy(plan)");
         }
+    }
+
+    // Host <-> device copy on the solver's stream, complete on return. Not hipMemcpy: a copy on
+    // the legacy stream fails while another thread captures a graph of its own solver.
+    void copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind,
+                   const char* what) const
+    {
+        check(hipMemcpyAsync(dst, src, bytes, kind, stream), what);
+        check(hipStreamSynchronize(stream), what);
     }
 
     void upload(const double* const comps[9], size_t n)
@@ -260,9 +268,8 @@ struct mhd_hip_solver {
         hipSetDevice(device);
         if (precision == MHD_HIP_FP64) {
             for (int c = 0; c < 9; ++c) {
-                check(hipMemcpy(static_cast<double*>(U) + c * plane(), comps[c], n * sizeof(double),
-                                hipMemcpyHostToDevice),
-                      "hipMemcpy(upload)");
+                copy_sync(static_cast<double*>(U) + c * plane(), comps[c], n * sizeof(double),
+                          hipMemcpyHostToDevice, "hipMemcpy(upload)");
             }
         } else {
             std::vector<float> stage(n);
@@ -270,9 +277,8 @@ struct mhd_hip_solver {
                 for (size_t k = 0; k < n; ++k) {
                     stage[k] = static_cast<float>(comps[c][k]);
                 }
-                check(hipMemcpy(static_cast<float*>(U) + c * plane(), stage.data(),
-                                n * sizeof(float), hipMemcpyHostToDevice),
-                      "hipMemcpy(upload)");
+                copy_sync(static_cast<float*>(U) + c * plane(), stage.data(), n * sizeof(float),
+                          hipMemcpyHostToDevice, "hipMemcpy(upload)");
             }
         }
     }
@@ -286,16 +292,14 @@ struct mhd_hip_solver {
         check(hipStreamSynchronize(stream), "hipStreamSynchronize");
         if (precision == MHD_HIP_FP64) {
             for (int c = 0; c < 9; ++c) {
-                check(hipMemcpy(comps[c], static_cast<double*>(U) + c * plane(), n * sizeof(double),
-                                hipMemcpyDeviceToHost),
-                      "hipMemcpy(download)");
+                copy_sync(comps[c], static_cast<double*>(U) + c * plane(), n * sizeof(double),
+                          hipMemcpyDeviceToHost, "hipMemcpy(download)");
             }
         } else {
             std::vector<float> stage(n);
             for (int c = 0; c < 9; ++c) {
-                check(hipMemcpy(stage.data(), static_cast<float*>(U) + c * plane(),
-                                n * sizeof(float), hipMemcpyDeviceToHost),
-                      "hipMemcpy(download)");
+                copy_sync(stage.data(), static_cast<float*>(U) + c * plane(), n * sizeof(float),
+                          hipMemcpyDeviceToHost, "hipMemcpy(download)");
                 for (size_t k = 0; k < n; ++k) {
                     comps[c][k] = static_cast<double>(stage[k]);
                 }
@@ -957,9 +961,8 @@ int mhd_hip_dt_log(mhd_hip_solver* s, double* out, int capacity, int* count)
         if (n > 0) {
             if (!out) throw InvalidArgument("null output buffer");
             hipSetDevice(s->device);
-            check(hipMemcpy(out, s->dt_log, static_cast<size_t>(n) * sizeof(double),
-                            hipMemcpyDeviceToHost),
-                  "hipMemcpy(dt_log)");
+            s->copy_sync(out, s->dt_log, static_cast<size_t>(n) * sizeof(double),
+                         hipMemcpyDeviceToHost, "hipMemcpy(dt_log)");
         }
     });
 }
