@@ -504,16 +504,13 @@ struct mhd_hip_batch {
     // most kHalfCells columns wide, half-wave tiles for that strip, each pairing a chunk of the
     // strip's lower half of rows (lanes 0..31) with the same chunk of its upper half
     // (lanes 32..63). f(strip, r0, r1, r0b, r1b); r0b < 0 marks a whole-wave tile.
-    // Half-wave tiles are made for the fp64 batch only: in the fp32 kernel the second tile
-    // kind costs the registers that keep its stage 2 free of spills.
     template <typename F>
-    void for_each_tile(const mhd_hip_instance& d, int rows, F&& f) const
+    static void for_each_tile(const mhd_hip_instance& d, int rows, F&& f)
     {
         const int ng = 2;
         const int full = d.nx / mhdhip::kStripCells;
         const int tail = d.nx - full * mhdhip::kStripCells;
-        const bool half =
-            precision == MHD_HIP_FP64 && tail > 0 && tail <= mhdhip::kHalfCells;
+        const bool half = tail > 0 && tail <= mhdhip::kHalfCells;
         const int wide = half ? full : (d.nx + mhdhip::kStripCells - 1) / mhdhip::kStripCells;
         for (int r0 = ng; r0 < ng + d.ny; r0 += rows) {
             for (int st = 0; st < wide; ++st) {

@@ -36,8 +36,14 @@ struct BatchTables {
     const int* ntiles;
 };
 
+// Waves per SIMD each batched stage kernel is compiled for: what its registers allow (fp32
+// ideal 128 VGPRs, non-ideal 168; fp64 ideal 256, non-ideal 512 with AGPRs). The fp32 ideal
+// kernel needs the bound: its march for both tile kinds lands one register above 128.
+template <typename R, int NI>
+constexpr int kBatchWavesPerEU = sizeof(R) == 4 ? (NI ? 3 : 4) : (NI ? 1 : 2);
+
 template <int STAGE, int LIM, int NI, typename R>
-__global__ __launch_bounds__(kBlock)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBatchWavesPerEU<R, NI>)))
 void k_batch_stage(BatchTables tb, int parity)
 {
     const int lane = threadIdx.x & (kWave - 1);
@@ -74,18 +80,10 @@ void k_batch_stage(BatchTables tb, int parity)
         const R* src = static_cast<const R*>(STAGE == 1 ? in.U : in.Us);
         const R* u0 = static_cast<const R*>(in.U);
         R* dst = static_cast<R*>(STAGE == 1 ? in.Us : in.U);
-        // half-wave tiles (r0b >= 0) come from the fp64 tables only; the fp32 kernel holds
-        // the whole-wave march alone
-        R local;
-        if (sizeof(R) == 4 || tile.r0b < 0) {
-            local = march_tile<STAGE, LIM, NI, 1, 0, 1, R>(src, u0, dst, nullptr, in.g, in.ph, st,
-                                                           tile.strip, tile.r0, tile.r1, 0,
-                                                           tile.r1, ring[wave], lane);
-        } else {
-            local = march_tile<STAGE, LIM, NI, 1, 1, 0, R>(
-                src, u0, dst, nullptr, in.g, in.ph, st, tile.strip, tile.r0, tile.r1,
-                tile.r0b - tile.r0, tile.r1b, ring[wave], lane);
-        }
+        const int dB = tile.r0b < 0 ? 0 : tile.r0b - tile.r0;
+        const R local = march_tile<STAGE, LIM, NI, 1, 2, 1, R>(
+            src, u0, dst, nullptr, in.g, in.ph, st, tile.strip, tile.r0, tile.r1, dB,
+            tile.r0b < 0 ? tile.r1 : tile.r1b, ring[wave], lane);
         if (STAGE == 2) {
             const R m = wave_max(local);
             if (lane == 0) {

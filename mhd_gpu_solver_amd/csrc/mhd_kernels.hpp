@@ -502,11 +502,12 @@ struct StageBuffers {
 // RK stage to its cells. `ring` is this wave's LDS slice. Returns the wave-local maximum CFL
 // signal speed of the updated fluid cells (stage 2; 0 in stage 1).
 //
-// HALF: the strip is a narrow one (at most kHalfCells columns, the tail of a grid) and the wave
-// works as two half-wave strips of 32 lanes, each with its own 2 halo lanes on either side.
-// Lanes 0..31 march rows [r0, r1), lanes 32..63 rows [r0 + dB, r1 + dB), of which only those
-// below r1b are updated. Shifts across lanes 31/32 feed only halo lanes, whose faces are never
-// used (the same argument as for the wave's edge lanes).
+// HALF 1: the strip is a narrow one (at most kHalfCells columns, the tail of a grid) and the
+// wave works as two half-wave strips of 32 lanes, each with its own 2 halo lanes on either
+// side. Lanes 0..31 march rows [r0, r1), lanes 32..63 rows [r0 + dB, r1 + dB), of which only
+// those below r1b are updated. Shifts across lanes 31/32 feed only halo lanes, whose faces are
+// never used (the same argument as for the wave's edge lanes). HALF 2: either kind, chosen per
+// tile by dB > 0 (wave-uniform), with one copy of the march for both; HALF 0: whole-wave only.
 //
 // EARLY: stage 2 issues its loads of U for the updated row at the start of the row step
 // (latency overlapped with the faces, nine more live registers); otherwise at the update.
@@ -518,15 +519,17 @@ __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, 
 {
     const long long P = g.plane;
 
-    const bool upper = HALF && lane >= kWave / 2;
-    const int l = HALF ? (lane & (kWave / 2 - 1)) : lane;  // lane within its strip
-    const int row_off = upper ? dB : 0;                      // this lane's row offset
+    const bool halves = HALF == 1 || (HALF == 2 && dB > 0);  // wave-uniform
+    const bool upper = halves && lane >= kWave / 2;
+    const int l = halves ? (lane & (kWave / 2 - 1)) : lane;  // lane within its strip
+    const int row_off = upper ? dB : 0;                        // this lane's row offset
+    const int lane_off = row_off * g.nxt;                      // the same in cells
     const int gi = g.ng + strip * kStripCells - 2 + l;
     const int gic = min(gi, g.nxt - 1);
     const int col = REMAP ? ghost_source(gic, g.nx, g.ng, g.bc_x) : gic;
     const bool cell_lane =
-        l >= 2 && l < 2 + (HALF ? kHalfCells : kStripCells) && gi < g.ng + g.nx;
-    constexpr unsigned long long x_live = HALF ? kXFaceLanesHalf : kXFaceLanes;
+        l >= 2 && l < 2 + (halves ? kHalfCells : kStripCells) && gi < g.ng + g.nx;
+    const unsigned long long x_live = halves ? kXFaceLanesHalf : kXFaceLanes;
 
     const R gamma = static_cast<R>(ph.gamma);
     const R gm1 = static_cast<R>(ph.gm1);
@@ -545,9 +548,17 @@ __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, 
     const R half_dt = static_cast<R>(0.5 * st.dt);
 
     auto load_row = [&](int q, R (&u)[kNC]) {
-        const int ql = q + row_off;
-        const int qr = REMAP ? ghost_source(ql, g.ny, g.ng, g.bc_y) : ql;
-        const long long base = static_cast<long long>(qr) * g.nxt + col;
+        long long base;
+        if constexpr (HALF == 2) {
+            // both halves' rows are remapped as wave-uniform values; a lane selects its own
+            const int qa = REMAP ? ghost_source(q, g.ny, g.ng, g.bc_y) : q;
+            const int qb = REMAP ? ghost_source(q + dB, g.ny, g.ng, g.bc_y) : q + dB;
+            base = static_cast<long long>(qa) * g.nxt + (col + (upper ? (qb - qa) * g.nxt : 0));
+        } else {
+            const int ql = q + row_off;
+            const int qr = REMAP ? ghost_source(ql, g.ny, g.ng, g.bc_y) : ql;
+            base = static_cast<long long>(qr) * g.nxt + col;
+        }
 #pragma unroll
         for (int c = 0; c < kNC; ++c) {
             u[c] = src[c * P + base];
@@ -586,7 +597,7 @@ __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, 
         constexpr bool early_u0 = STAGE == 2 && EARLY;
         R U0r[kNC];
         if (early_u0 && c >= r0 + 1) {
-            const long long base = static_cast<long long>(c - 1 + row_off) * g.nxt + gic;
+            const long long base = static_cast<long long>(c - 1) * g.nxt + (gic + lane_off);
 #pragma unroll
             for (int k = 0; k < kNC; ++k) {
                 U0r[k] = u0[k * P + base];
@@ -678,7 +689,7 @@ __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, 
                             L[k] = R(0);
                         }
                     }
-                    const long long cell = static_cast<long long>(rl) * g.nxt + gi;
+                    const long long cell = static_cast<long long>(rr) * g.nxt + (gi + lane_off);
                     const int slot = slot_of(rr);
                     if (STAGE == 1) {
                         // include/mhd/field_ops.hpp:47-53
