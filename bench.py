@@ -291,8 +291,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["ensemble", "single", "serve"], default=None)
     ap.add_argument("--serve-requests", type=int, default=128)
-    ap.add_argument("--batch-rows", type=int, default=0,
-                    help="batch rows per wave: 0 load model, -1 timed once, >0 fixed")
+    ap.add_argument("--batch-rows", type=int, default=-1,
+                    help="batch rows per wave: -1 timed once on the ensemble (default: the "
+                         "instance set is static), 0 load model, >0 fixed")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="also run plain device copies of BYTES (counter scale for profiling)")
@@ -357,6 +358,7 @@ def main():
                                      f"k_batch_stage<{dom + 1}", 4, f"ensemble_{len(specs)}",
                                      "k_batch_stage")),
             "seg_rows": b32["seg_rows"], "persistent_blocks": b32["blocks"],
+            "row_choice": {-1: "timed once", 0: "load model"}.get(args.batch_rows, "fixed"),
             "drift_fp32_periodic_max": b32["drift"],
             "batched_fp64": {"value": round(b64["mcups"], 2),
                              "drift_periodic_max": b64["drift"],
