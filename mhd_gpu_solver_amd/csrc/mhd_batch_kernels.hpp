@@ -43,7 +43,8 @@ template <typename R, int NI>
 constexpr int kBatchWavesPerEU = sizeof(R) == 4 ? (NI ? 3 : 4) : (NI ? 1 : 2);
 
 template <int STAGE, int LIM, int NI, typename R>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kBatchWavesPerEU<R, NI>)))
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(kBatchWavesPerEU<R, NI>)))
 void k_batch_stage(BatchTables tb, int parity)
 {
     const int lane = threadIdx.x & (kWave - 1);
