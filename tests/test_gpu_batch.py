@@ -206,3 +206,39 @@ def test_batch_fp32_narrow_grids_tiling_independent_and_within_l1(hip, oracle):
             den = np.abs(ref).sum()
             if den > 0:
                 assert np.abs(out[c, 2:-2, 2:-2] - ref).sum() / den <= 1e-4, (name, nx, ny, c)
+
+
+def test_batch_engines_in_threads_equal_alone(hip):
+    """Two batch engines (fp32 and fp64, each with its own stream and captured graph) advanced
+    from two Python threads at once give the same bits and step counts as each advanced alone."""
+    import threading
+
+    specs = mixed_specs(hip, 6, seed=23)
+    states = [s.state() for s in specs]
+
+    def run(precision):
+        eng = hip.BatchEngine(precision, limiter=hip.SlopeLimiter.MC, hbm_budget=1 << 28)
+        ids = [eng.add(U, s.t_end, gamma=s.gamma, bc=s.bc) for s, U in zip(specs, states)]
+        eng.advance()
+        return [(eng.result(i)[0], eng.download(i)) for i in ids]
+
+    alone = {p: run(p) for p in ("fp32", "fp64")}
+    out, errors = {}, []
+
+    def work(p):
+        try:
+            out[p] = run(p)
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    threads = [threading.Thread(target=work, args=(p,)) for p in ("fp32", "fp64")]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    assert not errors, errors
+    for p in ("fp32", "fp64"):
+        assert p in out
+        for (ra, Ua), (rb, Ub) in zip(alone[p], out[p]):
+            assert ra == rb
+            assert np.array_equal(Ua, Ub)
