@@ -63,7 +63,9 @@ def params(m, gamma=5.0 / 3.0):
 def single_leg(m, n, precision, steps, warmup, U0):
     """`steps` timed steps of the single-run solver after `warmup`."""
     dev = make_device(m, n, n, precision)
-    dev.upload(U0)
+    t0 = time.perf_counter()
+    dev.upload(U0)                                   # full fp64 host state, ghosts included
+    upload_s = time.perf_counter() - t0
     dev.fill_boundaries(5.0 / 3.0)
     p = params(m)
     if warmup:
@@ -83,13 +85,19 @@ def single_leg(m, n, precision, steps, warmup, U0):
     rows = dev.seg_rows()
     dev.set_kernel_timing(False)
     U = np.zeros_like(U0)
+    t0 = time.perf_counter()
     dev.download(U)
+    download_s = time.perf_counter() - t0
     if not np.isfinite(U).all():
         raise RuntimeError(f"{precision}: non-finite state after the timed steps")
     cells = n * n
+    # what a caller handing over host buffers sees for one call of `steps` steps: upload,
+    # graph-path steps, download (MHDSolver.run moves the state once each way per call)
+    pcie = {"mcups": round(cells * steps / (upload_s + graph_s + download_s) / 1e6, 2),
+            "steps": steps, "upload_s": round(upload_s, 4), "download_s": round(download_s, 4)}
     return dict(mcups=cells * steps / wall / 1e6, ms_per_step=wall * 1e3 / steps,
                 graph_mcups=cells * steps / graph_s / 1e6, kernels=k, seg_rows=rows,
-                cells=cells, drift=drift)
+                cells=cells, drift=drift, pcie=pcie)
 
 
 def measured_traffic(kernel_prefix, elem_bytes, workload, kernel="k_stage"):
@@ -146,7 +154,9 @@ def ensemble_batched(m, specs, states, precision, steps, warmup, seg_rows=0):
         eng.advance()
         for i in ids:
             eng.remove(i)
-    ids = admit(steps)
+    t0 = time.perf_counter()
+    ids = admit(steps)                               # host fp64 states into the HBM arena
+    admit_s = time.perf_counter() - t0
     before = {i: eng.totals(i) for i, s in zip(ids, specs) if s.bc == (0, 0)}
     eng.set_kernel_timing(True)
     t0 = time.perf_counter()
@@ -173,6 +183,7 @@ def ensemble_batched(m, specs, states, precision, steps, warmup, seg_rows=0):
     return dict(mcups=cells * steps / wall / 1e6, ms_per_step=wall * 1e3 / steps,
                 graph_mcups=cells * steps / graph_s / 1e6, kernels=kernels, cells=cells,
                 seg_rows=stats["seg_rows"], blocks=stats["blocks"], results=results,
+                admit_s=admit_s,
                 drift={"mass": float(f"{drift[0]:.3e}"), "energy": float(f"{drift[1]:.3e}"),
                        "instances": len(before)})
 
@@ -360,6 +371,7 @@ def main():
             "seg_rows": b32["seg_rows"], "persistent_blocks": b32["blocks"],
             "row_choice": {-1: "timed once", 0: "load model"}.get(args.batch_rows, "fixed"),
             "drift_fp32_periodic_max": b32["drift"],
+            "admission_s": round(b32["admit_s"], 4),
             "batched_fp64": {"value": round(b64["mcups"], 2),
                              "drift_periodic_max": b64["drift"],
                              "ms_per_step": round(b64["ms_per_step"], 4),
@@ -398,7 +410,7 @@ def main():
             legs[prec] = {
                 "value": round(leg["mcups"], 2), "ms_per_step": round(leg["ms_per_step"], 4),
                 "graph_path_mcups": round(leg["graph_mcups"], 2), "seg_rows": leg["seg_rows"],
-                "drift": leg["drift"],
+                "drift": leg["drift"], "pcie_inclusive": leg["pcie"],
                 "roofline": roofline(leg["kernels"], leg["cells"], leg["ms_per_step"], eb,
                                      "k_stage",
                                      traffic=lambda dom, eb=eb: measured_traffic(
