@@ -214,6 +214,32 @@ def ensemble_plain(m, specs, states, steps):
     return dict(mcups=cells * steps / total / 1e6, seconds=total, results=results)
 
 
+def backend_compare(m, sizes, t_end, precisions=("fp64", "fp32")):
+    """The reference's backend-comparison speed table (tools/mhd_backend_compare.cpp:128-153,
+    509-531) on the hip backend: its Orszag-Tang (B0 = 1), MC limiter, to t_end; one
+    MHDSolver.run per size and precision, host<->device transfers of the state included,
+    after a warm-up solve of the same shape (run(min(t_end, 1e-4)) on a second solver)."""
+    out = {}
+    for prec in precisions:
+        rows = []
+        for n in sizes:
+            def make():
+                s = m.MHDSolver(n, n)
+                s.set_limiter_mc()
+                s.initialize("orszag_tang")
+                s.set_backend("hip", prec)
+                return s
+            make().run(min(t_end, 1e-4))
+            s = make()
+            t0 = time.perf_counter()
+            r = s.run(t_end)
+            secs = time.perf_counter() - t0
+            rows.append({"n": n, "steps": r.steps, "seconds": round(secs, 4),
+                         "mcups": round(n * n * r.steps / secs / 1e6, 1)})
+        out[prec] = rows
+    return out
+
+
 def rel_l1_by_field(a, b):
     i = (slice(2, -2), slice(2, -2))
     out = []
@@ -300,7 +326,9 @@ def main():
     ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["ensemble", "single", "serve"], default=None)
+    ap.add_argument("--only", choices=["ensemble", "single", "compare", "serve"], default=None)
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the reference's backend-comparison table (OT, MC, t_end 0.02)")
     ap.add_argument("--serve-requests", type=int, default=128)
     ap.add_argument("--batch-rows", type=int, default=-1,
                     help="batch rows per wave: -1 timed once on the ensemble (default: the "
@@ -421,6 +449,20 @@ def main():
             out.update({"value": legs["fp64"]["value"], "ms_per_step": legs["fp64"]["ms_per_step"],
                         "dtype": "f64", "config": {"workload": f"orszag_tang_{n}x{n}_single_run"},
                         "roofline": legs["fp64"]["roofline"], "cpu_baseline": None})
+    if not args.no_compare and args.only in (None, "compare"):
+        log("backend comparison: Orszag-Tang (B0 = 1), MC, t_end 0.02, transfers included")
+        cmp_ = backend_compare(m, (128, 256, 512, 1024), 0.02)
+        for prec, rows in cmp_.items():
+            log(f"compare {prec}: " + ", ".join(f"{r['n']}^2 {r['mcups']}" for r in rows))
+        out["backend_compare"] = {
+            "workload": "orszag_tang (reference IC, B0 = 1), MC, t_end 0.02, MHDSolver.run "
+                        "incl. host<->device transfers",
+            **cmp_}
+        if args.only == "compare":
+            top = cmp_["fp32"][-1]
+            out.update({"value": top["mcups"], "dtype": "f32", "ms_per_step": None,
+                        "config": {"workload": f"backend_compare_orszag_tang_{top['n']}"},
+                        "roofline": None, "cpu_baseline": None})
     if not args.no_serve and args.only in (None, "serve"):
         from mhd_gpu_solver_amd.serve import Server
         reqs = workloads.request_stream(args.serve_requests, rate=200.0, seed=7)
