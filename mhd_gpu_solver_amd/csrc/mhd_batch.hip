@@ -497,7 +497,8 @@ struct mhd_hip_batch {
         dirty = true;
     }
 
-    static constexpr int kRowCandidates[] = {8, 12, 16, 20, 24, 28, 32, 40, 48, 64, 96, 128};
+    static constexpr int kRowCandidates[] = {2,  4,  6,  8,  12, 16, 20, 24,
+                                             28, 32, 40, 48, 64, 96, 128};
 
     // The tiles of one instance for `rows` rows per wave, in launch order: whole-wave tiles
     // over its 60-column strips, row segment by row segment; then, when its last strip is at

@@ -467,7 +467,9 @@ struct mhd_hip_solver {
         check(hipEventCreate(&e0), "hipEventCreate");
         check(hipEventCreate(&e1), "hipEventCreate");
         // candidates: fixed lengths plus the lengths that make the wave count just fill
-        // k whole rounds of the device's concurrent wave slots (no half-empty last round)
+        // k whole rounds of the device's concurrent wave slots (no half-empty last round);
+        // grids too small to fill the slots with 8-row segments also try 2 to 6 rows, since
+        // there a step lasts as long as one wave's march
         std::vector<int> cands = {8, 16, 24, 32, 48, 64, 96, 128};
         {
             hipDeviceProp_t prop;
@@ -479,7 +481,12 @@ struct mhd_hip_solver {
             for (int k = 1; k <= 6; ++k) {
                 const long long segs = std::max<long long>(1, k * slots / strips);
                 const int rows = static_cast<int>((grid.ny + segs - 1) / segs);
-                if (rows >= 4 && rows <= 512) {
+                if (rows >= 2 && rows <= 512) {
+                    cands.push_back(rows);
+                }
+            }
+            if (strips * ((grid.ny + 7) / 8) < slots) {
+                for (const int rows : {2, 3, 4, 6}) {
                     cands.push_back(rows);
                 }
             }
