@@ -214,11 +214,12 @@ def ensemble_plain(m, specs, states, steps):
     return dict(mcups=cells * steps / total / 1e6, seconds=total, results=results)
 
 
-def backend_compare(m, sizes, t_end, precisions=("fp64", "fp32")):
+def backend_compare(m, sizes, t_end, precisions=("fp64", "fp32"), repeats=3):
     """The reference's backend-comparison speed table (tools/mhd_backend_compare.cpp:128-153,
-    509-531) on the hip backend: its Orszag-Tang (B0 = 1), MC limiter, to t_end; one
-    MHDSolver.run per size and precision, host<->device transfers of the state included,
-    after a warm-up solve of the same shape (run(min(t_end, 1e-4)) on a second solver)."""
+    509-531) on the hip backend: its Orszag-Tang (B0 = 1), MC limiter, to t_end; MHDSolver.run
+    per size and precision, host<->device transfers of the state included, after a warm-up
+    solve of the same shape (run(min(t_end, 1e-4)) on a second solver). The median of
+    `repeats` runs, as the reference's median_rate takes (:290-300): runs of a few ms vary."""
     out = {}
     for prec in precisions:
         rows = []
@@ -230,10 +231,13 @@ def backend_compare(m, sizes, t_end, precisions=("fp64", "fp32")):
                 s.set_backend("hip", prec)
                 return s
             make().run(min(t_end, 1e-4))
-            s = make()
-            t0 = time.perf_counter()
-            r = s.run(t_end)
-            secs = time.perf_counter() - t0
+            times = []
+            for _ in range(repeats):
+                s = make()
+                t0 = time.perf_counter()
+                r = s.run(t_end)
+                times.append(time.perf_counter() - t0)
+            secs = sorted(times)[len(times) // 2]
             rows.append({"n": n, "steps": r.steps, "seconds": round(secs, 4),
                          "mcups": round(n * n * r.steps / secs / 1e6, 1)})
         out[prec] = rows

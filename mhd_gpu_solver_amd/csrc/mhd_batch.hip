@@ -497,8 +497,12 @@ struct mhd_hip_batch {
         dirty = true;
     }
 
-    static constexpr int kRowCandidates[] = {2,  4,  6,  8,  12, 16, 20, 24,
-                                             28, 32, 40, 48, 64, 96, 128};
+    // Row lengths the load model weighs. The timed mode also tries 2, 4 and 6: the model,
+    // which counts each wave's own work only, would pick those for sets that fit in one
+    // round of the grid, where the extra halo rows cost more than the shorter march saves.
+    static constexpr int kRowCandidates[] = {8, 12, 16, 20, 24, 28, 32, 40, 48, 64, 96, 128};
+    static constexpr int kTimedRowCandidates[] = {2,  4,  6,  8,  12, 16, 20, 24,
+                                                  28, 32, 40, 48, 64, 96, 128};
 
     // The tiles of one instance for `rows` rows per wave, in launch order: whole-wave tiles
     // over its 60-column strips, row segment by row segment; then, when its last strip is at
@@ -652,7 +656,7 @@ struct mhd_hip_batch {
     {
         int best = 32;
         float best_ms = 1e30f;
-        for (const int rows : kRowCandidates) {
+        for (const int rows : kTimedRowCandidates) {
             build_tables(rows);
             check(mhdhip::launch_batch_stage<R>(1, limiter, nonideal, tables(), 0, blocks, stream),
                   "tune");
