@@ -523,7 +523,9 @@ __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, 
     const bool upper = halves && lane >= kWave / 2;
     const int l = halves ? (lane & (kWave / 2 - 1)) : lane;  // lane within its strip
     const int row_off = upper ? dB : 0;                        // this lane's row offset
-    const int lane_off = row_off * g.nxt;                      // the same in cells
+    // the same in cells; dB <= ny / 2 + 1 keeps it (and the row difference in load_row) below
+    // half a plane plus a row, within int for any grid whose two states fit in HBM
+    const int lane_off = row_off * g.nxt;
     const int gi = g.ng + strip * kStripCells - 2 + l;
     const int gic = min(gi, g.nxt - 1);
     const int col = REMAP ? ghost_source(gic, g.nx, g.ng, g.bc_x) : gic;
