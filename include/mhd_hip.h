@@ -277,8 +277,9 @@ int mhd_hip_batch_totals(mhd_hip_batch* b, int id, double sums[9]);
 
 /* Rows per wave for the batch kernel. 0 (default): chosen for the current stepping set,
  * whenever it changes, by replaying the kernel's round-robin tile assignment and minimising
- * the busiest wave's rows; -1: timed over the candidates once, on the first advance's set;
- * > 0: fixed. */
+ * the busiest wave's rows (suits a set that changes, as in serving); -1: one stage-1 pass per
+ * candidate is timed once, on the first advance's set (the better choice for a set that stays
+ * the same); > 0: fixed. Results do not depend on this value. */
 int mhd_hip_batch_set_seg_rows(mhd_hip_batch* b, int rows);
 
 int mhd_hip_batch_info(const mhd_hip_batch* b, mhd_hip_batch_stats* stats);
