@@ -10,6 +10,7 @@
 #include "mhd_hip.h"
 #include "mhd_host.hpp"
 #include "mhd_launch.hpp"
+#include "mhd_transfer.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -117,6 +118,8 @@ struct mhd_hip_solver {
     mhdhip::DD* totals_partial = nullptr;  // compensated-sum scratch, allocated on first use
     double* totals_out = nullptr;
 
+    mhdhip::StagedCopier staging;  // pinned chunk buffers for large uploads and downloads
+
     ~mhd_hip_solver() { release(); }
 
     void release()
@@ -136,6 +139,7 @@ struct mhd_hip_solver {
         if (totals_out) hipFree(totals_out);
         totals_partial = nullptr;
         totals_out = nullptr;
+        staging.release();
         for (hipEvent_t e : timing_events) hipEventDestroy(e);
         timing_events.clear();
         if (ev_start) hipEventDestroy(ev_start);
@@ -266,6 +270,14 @@ struct mhd_hip_solver {
             throw InvalidArgument("upload: size does not match the full SoA layout");
         }
         hipSetDevice(device);
+        if (9 * n * elem >= mhdhip::StagedCopier::kMinBytes) {
+            if (precision == MHD_HIP_FP64) {
+                staging.upload(static_cast<double*>(U), plane(), comps, 9, n, stream);
+            } else {
+                staging.upload(static_cast<float*>(U), plane(), comps, 9, n, stream);
+            }
+            return;
+        }
         if (precision == MHD_HIP_FP64) {
             for (int c = 0; c < 9; ++c) {
                 copy_sync(static_cast<double*>(U) + c * plane(), comps[c], n * sizeof(double),
@@ -290,6 +302,14 @@ struct mhd_hip_solver {
         }
         hipSetDevice(device);
         check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        if (9 * n * elem >= mhdhip::StagedCopier::kMinBytes) {
+            if (precision == MHD_HIP_FP64) {
+                staging.download(comps, static_cast<const double*>(U), plane(), 9, n, stream);
+            } else {
+                staging.download(comps, static_cast<const float*>(U), plane(), 9, n, stream);
+            }
+            return;
+        }
         if (precision == MHD_HIP_FP64) {
             for (int c = 0; c < 9; ++c) {
                 copy_sync(comps[c], static_cast<double*>(U) + c * plane(), n * sizeof(double),
