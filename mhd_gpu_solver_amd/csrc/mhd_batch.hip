@@ -10,6 +10,7 @@
 #include "mhd_abi.hpp"
 #include "mhd_hip.h"
 #include "mhd_launch.hpp"
+#include "mhd_transfer.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -400,11 +401,7 @@ struct mhd_hip_batch {
         ensure_stage(9 * n * sizeof(R));
         check(hipStreamSynchronize(stream), "hipStreamSynchronize");  // staging reuse
         R* st = reinterpret_cast<R*>(h_stage);
-        for (int c = 0; c < 9; ++c) {
-            for (size_t k = 0; k < n; ++k) {
-                st[c * n + k] = static_cast<R>(comps[c][k]);
-            }
-        }
+        mhdhip::gather_planes(st, comps, 9, n);
         char* base = static_cast<char*>(arena_base) + s.off;
         check(hipMemcpyAsync(base, st, 9 * n * sizeof(R), hipMemcpyHostToDevice, stream),
               "hipMemcpyAsync(upload)");
@@ -823,12 +820,7 @@ struct mhd_hip_batch {
         check(hipMemcpyAsync(h_stage, u, 9 * n * sizeof(R), hipMemcpyDeviceToHost, stream),
               "hipMemcpyAsync(download)");
         check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-        const R* st = reinterpret_cast<const R*>(h_stage);
-        for (int c = 0; c < 9; ++c) {
-            for (size_t k = 0; k < n; ++k) {
-                comps[c][k] = static_cast<double>(st[c * n + k]);
-            }
-        }
+        mhdhip::scatter_planes(comps, reinterpret_cast<const R*>(h_stage), 9, n);
     }
 
     void totals(int id, double out[9])

@@ -111,6 +111,52 @@ private:
     bool stop_ = false;
 };
 
+// Packs `count` host planes of n elements into one contiguous buffer (gather) or unpacks it
+// (scatter), converting the element type; on the copy threads when the state is large.
+template <typename Body>
+void for_plane_spans(int count, size_t n, Body&& body)
+{
+    const size_t total = static_cast<size_t>(count) * n;
+    auto spans = [&](size_t lo, size_t hi) {
+        for (size_t j = lo; j < hi;) {
+            const size_t c = j / n;
+            const size_t k = j - c * n;
+            const size_t m = std::min(hi - j, n - k);
+            body(static_cast<int>(c), k, m);
+            j += m;
+        }
+    };
+    if (total < (size_t(1) << 20)) {
+        spans(0, total);
+    } else {
+        CopyPool::instance().run(total, spans);
+    }
+}
+
+template <typename To, typename From>
+void gather_planes(To* dst, const From* const* src, int count, size_t n)
+{
+    for_plane_spans(count, n, [&](int c, size_t k, size_t m) {
+        To* d = dst + static_cast<size_t>(c) * n + k;
+        const From* f = src[c] + k;
+        for (size_t i = 0; i < m; ++i) {
+            d[i] = static_cast<To>(f[i]);
+        }
+    });
+}
+
+template <typename To, typename From>
+void scatter_planes(To* const* dst, const From* src, int count, size_t n)
+{
+    for_plane_spans(count, n, [&](int c, size_t k, size_t m) {
+        To* d = dst[c] + k;
+        const From* f = src + static_cast<size_t>(c) * n + k;
+        for (size_t i = 0; i < m; ++i) {
+            d[i] = static_cast<To>(f[i]);
+        }
+    });
+}
+
 // Two pinned chunk buffers and their copy-done events, owned by one solver.
 class StagedCopier {
 public:
