@@ -8,6 +8,11 @@ recorded. New requests join between replays while the others keep running, so th
 overlaps on the device. No counterpart exists in the reference, which solves one grid per
 call (include/mhd/solver_api.hpp:294-321).
 
+Host preparation: the initial state of an arrived request is built on a few host threads while
+the device runs the current replay (up to ``prepare_ahead`` requests ahead of admission), so
+the device does not wait on it. Admission keeps FIFO order: while solves are running, a queue
+head whose state is not ready yet waits for the next replay.
+
 Scheduling: with ``policy="all"`` every admitted solve steps in every replay (processor
 sharing). With ``policy="srpt"`` admitted solves stay resident in HBM but each replay steps
 only those with the least estimated remaining work (cells x remaining steps, the steps taken
@@ -18,7 +23,9 @@ from __future__ import annotations
 
 import collections
 import dataclasses
+import itertools
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -95,13 +102,21 @@ class Server:
         self.work_cap_cells = int(work_cap_cells)
         self.engine.set_schedule(policy, self.work_cap_cells)
 
-    def run(self, requests, realtime=True, keep_states=False, clock=time.perf_counter):
+    def run(self, requests, realtime=True, keep_states=False, clock=time.perf_counter,
+            prepare_workers=4, prepare_ahead=16):
         """Serve `requests` (workloads.Spec with .arrival seconds, .t_end) to completion.
         realtime: arrivals follow the wall clock from the start of the call; otherwise every
-        request is available at once."""
+        request is available at once. Initial states are built by `prepare_workers` host
+        threads, at most `prepare_ahead` arrived requests ahead of admission."""
+        with ThreadPoolExecutor(max_workers=max(1, prepare_workers)) as prep:
+            return self._run(requests, realtime, keep_states, clock, prep,
+                             max(1, prepare_ahead))
+
+    def _run(self, requests, realtime, keep_states, clock, prep, prepare_ahead):
         order = sorted(range(len(requests)), key=lambda k: requests[k].arrival)
         pending = collections.deque(order)
         queue = collections.deque()
+        states = {}  # request index -> future of its initial state
         active = {}
         done = []
         t0 = clock()
@@ -110,19 +125,27 @@ class Server:
         while pending or queue or active:
             now = clock() - t0
             while pending and (not realtime or requests[pending[0]].arrival <= now):
-                queue.append(pending.popleft())
-            while queue:
-                k = queue[0]
+                k = pending.popleft()
                 req = requests[k]
                 if instance_bytes(self.precision, req.n, req.n) > self.hbm_budget:
                     raise ValueError(f"request {k} ({req.n}^2) exceeds the whole HBM budget")
+                queue.append(k)
+            for k in itertools.islice(queue, prepare_ahead):
+                if k not in states:
+                    states[k] = prep.submit(requests[k].state)
+            while queue:
+                k = queue[0]
+                req = requests[k]
+                if active and not states[k].done():
+                    break  # the device has work; admit this one after the next replay
                 try:
-                    iid = self.engine.add(req.state(), req.t_end, cfl=self.cfl, gamma=req.gamma,
-                                          glm_alpha=self.glm_alpha, bc=req.bc,
+                    iid = self.engine.add(states[k].result(), req.t_end, cfl=self.cfl,
+                                          gamma=req.gamma, glm_alpha=self.glm_alpha, bc=req.bc,
                                           max_steps=req.max_steps)
                 except CapacityFull:
                     break
                 queue.popleft()
+                del states[k]
                 active[iid] = (k, clock() - t0)
             max_conc = max(max_conc, len(active))
             if not active:
