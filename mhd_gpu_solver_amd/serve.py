@@ -61,6 +61,7 @@ class ServeReport:
     makespan: float
     max_concurrent: int
     replays: int
+    advance_s: float = 0.0  # wall time inside the engine's replays (the device's busy time)
 
     def latencies_ms(self):
         return np.array([c.latency for c in self.completed if c.error is None]) * 1e3
@@ -79,6 +80,7 @@ class ServeReport:
             "cell_updates": cu,
             "max_concurrent": self.max_concurrent,
             "replays": self.replays,
+            "advance_s": round(self.advance_s, 3),
         }
 
 
@@ -122,6 +124,7 @@ class Server:
         t0 = clock()
         max_conc = 0
         replays = 0
+        advance_s = 0.0
         while pending or queue or active:
             now = clock() - t0
             while pending and (not realtime or requests[pending[0]].arrival <= now):
@@ -152,7 +155,9 @@ class Server:
                 if pending and realtime:
                     time.sleep(max(0.0, requests[pending[0]].arrival - (clock() - t0)))
                 continue
+            t_adv = clock()
             self.engine.advance(max_replays=1)
+            advance_s += clock() - t_adv
             replays += 1
             for iid in self.engine.poll_finished():
                 k, adm = active.pop(iid)
@@ -168,4 +173,4 @@ class Server:
                 self.engine.remove(iid)
                 done.append(Completed(k, req.n, req.ic, req.arrival if realtime else 0.0, adm,
                                       clock() - t0, steps, t, state, err))
-        return ServeReport(done, clock() - t0, max_conc, replays)
+        return ServeReport(done, clock() - t0, max_conc, replays, advance_s)
