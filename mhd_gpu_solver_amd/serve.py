@@ -85,9 +85,11 @@ class ServeReport:
 
 
 class Server:
-    # cells stepping per replay under srpt: about three rounds of a full persistent grid on
-    # MI355X at 32 rows per wave (256 CUs x 3 blocks x 4 waves x 60 x 32 cells = 5.9M)
-    DEFAULT_WORK_CAP = 16 << 20
+    # cells stepping per replay under srpt. On MI355X (4096 fp32 wave slots, 60 columns each)
+    # 8M is about one round of the persistent grid at 32 rows per wave: the request stream of
+    # 128 solves at 200/s keeps its full throughput down to about 6M (4M: -5 %), while the
+    # median latency falls with the cap (16M: 1.25 s, 8M: 0.74 s, 6M: 0.60 s).
+    DEFAULT_WORK_CAP = 8 << 20
 
     def __init__(self, precision="fp32", limiter=SlopeLimiter.MC, hbm_budget=32 << 30,
                  max_instances=1024, cfl=0.4, glm_alpha=0.1, policy="srpt",
