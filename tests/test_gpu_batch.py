@@ -1,6 +1,5 @@
 """Batched engine: every instance gives the oracle's fp64 bits, its result is the same alone
 and in any mix (fp32 included), admission honours the HBM budget, and tiling does not matter."""
-import math
 
 import numpy as np
 import pytest

@@ -13,7 +13,7 @@ import math
 import numpy as np
 import pytest
 
-from checkers import MC, MINMOD, OUTFLOW, PERIODIC, FIELDS, make_grid, make_params
+from checkers import MINMOD, OUTFLOW, FIELDS, make_grid, make_params
 
 pytestmark = pytest.mark.gpu
 

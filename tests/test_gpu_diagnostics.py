@@ -64,7 +64,7 @@ def test_device_max_div_b_equals_oracle(hip, oracle, precision):
     """Device max |div B| after a solve equals the oracle's on the downloaded state, bit for
     bit (fp64 arithmetic from the stored values in both precisions); on the OT 32^2 golden
     case it also reproduces the golden's recorded value."""
-    from checkers import MC, make_grid, make_params
+    from checkers import make_grid
     s = hip.MHDSolver(32, 32)
     s.set_backend("hip", precision)
     s.set_limiter_mc()

@@ -10,7 +10,7 @@ import pathlib
 import numpy as np
 import pytest
 
-from checkers import MC, MINMOD, OUTFLOW, PERIODIC, FIELDS, make_grid, make_params
+from checkers import FIELDS, make_grid, make_params
 
 GOLD = pathlib.Path(__file__).parent / "golden"
 

@@ -10,7 +10,7 @@ import pytest
 
 import mhd_gpu_solver_amd as m
 from mhd_gpu_solver_amd import _lib
-from checkers import OUTFLOW, PERIODIC, interior, make_grid
+from checkers import interior, make_grid
 
 ROOT = pathlib.Path(__file__).resolve().parent.parent
 

@@ -1,5 +1,4 @@
 """Seeded workload generators (host only)."""
-import math
 
 import numpy as np
 
