@@ -175,25 +175,25 @@ struct mhd_hip_batch {
 
     ~mhd_hip_batch()
     {
-        if (device >= 0) hipSetDevice(device);
-        if (stream) hipStreamSynchronize(stream);
-        if (graph) hipGraphExecDestroy(graph);
-        if (arena_base) hipFree(arena_base);
-        if (d_insts) hipFree(d_insts);
-        if (d_ctrl) hipFree(d_ctrl);
-        if (d_tiles) hipFree(d_tiles);
-        if (d_ntiles) hipFree(d_ntiles);
-        if (h_insts) hipHostFree(h_insts);
-        if (h_ctrl) hipHostFree(h_ctrl);
-        if (h_tiles) hipHostFree(h_tiles);
-        if (h_ntiles) hipHostFree(h_ntiles);
-        if (h_stage) hipHostFree(h_stage);
-        if (totals_partial) hipFree(totals_partial);
-        if (totals_out) hipFree(totals_out);
-        for (hipEvent_t e : tev) hipEventDestroy(e);
-        if (ev0) hipEventDestroy(ev0);
-        if (ev1) hipEventDestroy(ev1);
-        if (stream) hipStreamDestroy(stream);
+        if (device >= 0) (void)hipSetDevice(device);
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (graph) (void)hipGraphExecDestroy(graph);
+        if (arena_base) (void)hipFree(arena_base);
+        if (d_insts) (void)hipFree(d_insts);
+        if (d_ctrl) (void)hipFree(d_ctrl);
+        if (d_tiles) (void)hipFree(d_tiles);
+        if (d_ntiles) (void)hipFree(d_ntiles);
+        if (h_insts) (void)hipHostFree(h_insts);
+        if (h_ctrl) (void)hipHostFree(h_ctrl);
+        if (h_tiles) (void)hipHostFree(h_tiles);
+        if (h_ntiles) (void)hipHostFree(h_ntiles);
+        if (h_stage) (void)hipHostFree(h_stage);
+        if (totals_partial) (void)hipFree(totals_partial);
+        if (totals_out) (void)hipFree(totals_out);
+        for (hipEvent_t e : tev) (void)hipEventDestroy(e);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
     }
 
     // One replay's worth of steps, launched eagerly with an event pair around each stage
@@ -302,8 +302,8 @@ struct mhd_hip_batch {
         }
         const size_t cap = std::max(n, tile_cap * 2);
         check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-        if (d_tiles) hipFree(d_tiles);
-        if (h_tiles) hipHostFree(h_tiles);
+        if (d_tiles) (void)hipFree(d_tiles);
+        if (h_tiles) (void)hipHostFree(h_tiles);
         check(hipMalloc(reinterpret_cast<void**>(&d_tiles), sizeof(BatchTile) * cap),
               "hipMalloc(tiles)");
         check(hipHostMalloc(reinterpret_cast<void**>(&h_tiles), sizeof(BatchTile) * cap, 0),
@@ -311,7 +311,7 @@ struct mhd_hip_batch {
         tile_cap = cap;
         h_tile_cap = cap;
         if (graph) {
-            hipGraphExecDestroy(graph);
+            (void)hipGraphExecDestroy(graph);
             graph = nullptr;
         }
         dirty = true;
@@ -322,7 +322,7 @@ struct mhd_hip_batch {
         if (bytes <= h_stage_bytes) {
             return;
         }
-        if (h_stage) hipHostFree(h_stage);
+        if (h_stage) (void)hipHostFree(h_stage);
         h_stage = nullptr;
         check(hipHostMalloc(reinterpret_cast<void**>(&h_stage), bytes, 0), "hipHostMalloc(stage)");
         h_stage_bytes = bytes;
@@ -458,7 +458,7 @@ struct mhd_hip_batch {
             throw Full("HBM budget exhausted (" + std::to_string(arena.used()) + " of " +
                        std::to_string(arena.capacity()) + " bytes in use)");
         }
-        hipSetDevice(device);
+        check(hipSetDevice(device), "hipSetDevice");
         Slot& s = slots[si];
         s = Slot{};
         s.used = true;
@@ -486,7 +486,7 @@ struct mhd_hip_batch {
     void remove(int id)
     {
         Slot& s = slot_of(id);
-        hipSetDevice(device);
+        check(hipSetDevice(device), "hipSetDevice");
         // in-flight replays may still read this slot's state: release only after them
         check(hipStreamSynchronize(stream), "hipStreamSynchronize");
         arena.release(s.off, s.bytes);
@@ -636,13 +636,13 @@ struct mhd_hip_batch {
                       "batch stage2");
             }
         } catch (...) {
-            hipStreamEndCapture(stream, &g);
-            if (g) hipGraphDestroy(g);
+            (void)hipStreamEndCapture(stream, &g);
+            if (g) (void)hipGraphDestroy(g);
             throw;
         }
         check(hipStreamEndCapture(stream, &g), "EndCapture");
         const hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
-        hipGraphDestroy(g);
+        (void)hipGraphDestroy(g);
         check(e, "hipGraphInstantiate");
     }
 
@@ -761,7 +761,7 @@ struct mhd_hip_batch {
     template <typename R>
     int advance_t(int max_replays)
     {
-        hipSetDevice(device);
+        check(hipSetDevice(device), "hipSetDevice");
         check(hipEventRecord(ev0, stream), "hipEventRecord");
         int active = 0;
         for (int rep = 0; max_replays < 0 || rep < max_replays; ++rep) {
@@ -826,7 +826,7 @@ struct mhd_hip_batch {
     void totals(int id, double out[9])
     {
         const Slot& s = slot_of(id);
-        hipSetDevice(device);
+        check(hipSetDevice(device), "hipSetDevice");
         if (!totals_partial) {
             check(hipMalloc(reinterpret_cast<void**>(&totals_partial),
                             sizeof(mhdhip::DD) * 9 * mhdhip::kTotalsBlocks),
@@ -871,7 +871,7 @@ struct mhd_hip_batch {
     double max_div_b(int id)
     {
         const Slot& s = slot_of(id);
-        hipSetDevice(device);
+        check(hipSetDevice(device), "hipSetDevice");
         if (!totals_out) {
             check(hipMalloc(reinterpret_cast<void**>(&totals_out), 9 * sizeof(double)),
                   "hipMalloc(totals)");
@@ -885,7 +885,7 @@ struct mhd_hip_batch {
         if (n != state_elems(s.desc.nx, s.desc.ny)) {
             throw InvalidArgument("download: size does not match the instance's full layout");
         }
-        hipSetDevice(device);
+        check(hipSetDevice(device), "hipSetDevice");
         if (precision == MHD_HIP_FP64) {
             download_t<double>(s, comps);
         } else {

@@ -125,26 +125,26 @@ struct mhd_hip_solver {
     void release()
     {
         if (device >= 0) {
-            hipSetDevice(device);
+            (void)hipSetDevice(device);
         }
-        if (graph_exec) hipGraphExecDestroy(graph_exec);
-        if (U) hipFree(U);
-        if (Us) hipFree(Us);
-        if (ctrl) hipFree(ctrl);
-        if (ctrl_host) hipHostFree(ctrl_host);
-        if (dt_log) hipFree(dt_log);
-        if (mask_dev) hipFree(mask_dev);
-        if (plan_dev) hipFree(plan_dev);
-        if (totals_partial) hipFree(totals_partial);
-        if (totals_out) hipFree(totals_out);
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (U) (void)hipFree(U);
+        if (Us) (void)hipFree(Us);
+        if (ctrl) (void)hipFree(ctrl);
+        if (ctrl_host) (void)hipHostFree(ctrl_host);
+        if (dt_log) (void)hipFree(dt_log);
+        if (mask_dev) (void)hipFree(mask_dev);
+        if (plan_dev) (void)hipFree(plan_dev);
+        if (totals_partial) (void)hipFree(totals_partial);
+        if (totals_out) (void)hipFree(totals_out);
         totals_partial = nullptr;
         totals_out = nullptr;
         staging.release();
-        for (hipEvent_t e : timing_events) hipEventDestroy(e);
+        for (hipEvent_t e : timing_events) (void)hipEventDestroy(e);
         timing_events.clear();
-        if (ev_start) hipEventDestroy(ev_start);
-        if (ev_stop) hipEventDestroy(ev_stop);
-        if (stream) hipStreamDestroy(stream);
+        if (ev_start) (void)hipEventDestroy(ev_start);
+        if (ev_stop) (void)hipEventDestroy(ev_stop);
+        if (stream) (void)hipStreamDestroy(stream);
         graph_exec = nullptr;
         U = Us = nullptr;
         ctrl = nullptr;
@@ -217,7 +217,7 @@ struct mhd_hip_solver {
         if (!valid(bx) || !valid(by)) {
             throw InvalidArgument("unknown boundary condition type");
         }
-        hipSetDevice(device);
+        check(hipSetDevice(device), "hipSetDevice");
         bc_x = bx;
         bc_y = by;
         geom.bc_x = bx;
@@ -244,7 +244,7 @@ struct mhd_hip_solver {
         std::vector<long long> plan;
         plan_count = static_cast<int>(mhdhip::wall_plan(grid, mask, plan));
         if (plan_dev) {
-            hipFree(plan_dev);
+            (void)hipFree(plan_dev);
             plan_dev = nullptr;
         }
         if (plan_count > 0) {
@@ -269,7 +269,7 @@ struct mhd_hip_solver {
         if (n != plane()) {
             throw InvalidArgument("upload: size does not match the full SoA layout");
         }
-        hipSetDevice(device);
+        check(hipSetDevice(device), "hipSetDevice");
         if (9 * n * elem >= mhdhip::StagedCopier::kMinBytes) {
             if (precision == MHD_HIP_FP64) {
                 staging.upload(static_cast<double*>(U), plane(), comps, 9, n, stream);
@@ -300,7 +300,7 @@ struct mhd_hip_solver {
         if (n != plane()) {
             throw InvalidArgument("download: size does not match the full SoA layout");
         }
-        hipSetDevice(device);
+        check(hipSetDevice(device), "hipSetDevice");
         check(hipStreamSynchronize(stream), "hipStreamSynchronize");
         if (9 * n * elem >= mhdhip::StagedCopier::kMinBytes) {
             if (precision == MHD_HIP_FP64) {
@@ -547,12 +547,12 @@ struct mhd_hip_solver {
                 }
             }
         } catch (...) {
-            hipEventDestroy(e0);
-            hipEventDestroy(e1);
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
             throw;
         }
-        hipEventDestroy(e0);
-        hipEventDestroy(e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
         std::lock_guard<std::mutex> lk(g_tune_mutex);
         g_tuned[key] = best;
         return best;
@@ -561,7 +561,7 @@ struct mhd_hip_solver {
     void invalidate_graph()
     {
         if (graph_exec) {
-            hipGraphExecDestroy(graph_exec);
+            (void)hipGraphExecDestroy(graph_exec);
             graph_exec = nullptr;
         }
         graph_key.clear();
@@ -592,13 +592,13 @@ struct mhd_hip_solver {
                 enqueue_step<R>(ph, tl, limiter, ni, s & 1);
             }
         } catch (...) {
-            hipStreamEndCapture(stream, &graph);
-            if (graph) hipGraphDestroy(graph);
+            (void)hipStreamEndCapture(stream, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
             throw;
         }
         check(hipStreamEndCapture(stream, &graph), "EndCapture");
         const hipError_t e = hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0);
-        hipGraphDestroy(graph);
+        (void)hipGraphDestroy(graph);
         check(e, "hipGraphInstantiate");
         graph_key = key;
     }
@@ -618,7 +618,7 @@ struct mhd_hip_solver {
         if (max_steps < 0) {
             throw InvalidArgument("max_steps must be non-negative");
         }
-        hipSetDevice(device);
+        check(hipSetDevice(device), "hipSetDevice");
         const Physics ph = physics(p, t_end, max_steps);
         const bool ni = nonideal_any(p);
         R* u = static_cast<R*>(U);
@@ -678,15 +678,15 @@ struct mhd_hip_solver {
                 }
                 check(hipStreamSynchronize(stream), "hipStreamSynchronize");
             } catch (...) {
-                hipStreamSynchronize(stream);
-                hipHostFree(host_slots);
-                hipEventDestroy(done[0]);
-                hipEventDestroy(done[1]);
+                (void)hipStreamSynchronize(stream);
+                (void)hipHostFree(host_slots);
+                (void)hipEventDestroy(done[0]);
+                (void)hipEventDestroy(done[1]);
                 throw;
             }
-            hipHostFree(host_slots);
-            hipEventDestroy(done[0]);
-            hipEventDestroy(done[1]);
+            (void)hipHostFree(host_slots);
+            (void)hipEventDestroy(done[0]);
+            (void)hipEventDestroy(done[1]);
         }
         // final ghost + wall fill: include/mhd/time_integrator.hpp:101-104
         const Gate open{ctrl, 0.0, 0, -1};
@@ -750,7 +750,7 @@ struct mhd_hip_solver {
 
     void fill_boundaries(double gamma, int what)
     {
-        hipSetDevice(device);
+        check(hipSetDevice(device), "hipSetDevice");
         if (precision == MHD_HIP_FP64) {
             fill_boundaries_t<double>(gamma, what);
         } else {
@@ -761,7 +761,7 @@ struct mhd_hip_solver {
 
     void totals(double out[9])
     {
-        hipSetDevice(device);
+        check(hipSetDevice(device), "hipSetDevice");
         if (!totals_partial) {
             check(hipMalloc(reinterpret_cast<void**>(&totals_partial),
                             sizeof(mhdhip::DD) * 9 * mhdhip::kTotalsBlocks),
@@ -785,7 +785,7 @@ struct mhd_hip_solver {
 
     double max_div_b()
     {
-        hipSetDevice(device);
+        check(hipSetDevice(device), "hipSetDevice");
         if (!totals_out) {
             check(hipMalloc(reinterpret_cast<void**>(&totals_out), 9 * sizeof(double)),
                   "hipMalloc(totals)");
@@ -815,9 +815,9 @@ struct mhd_hip_solver {
         if (cap < 0) {
             throw InvalidArgument("dt log capacity must be non-negative");
         }
-        hipSetDevice(device);
+        check(hipSetDevice(device), "hipSetDevice");
         if (dt_log) {
-            hipFree(dt_log);
+            (void)hipFree(dt_log);
             dt_log = nullptr;
         }
         dt_cap = cap;
@@ -987,7 +987,7 @@ int mhd_hip_dt_log(mhd_hip_solver* s, double* out, int capacity, int* count)
         *count = std::max(0, n);
         if (n > 0) {
             if (!out) throw InvalidArgument("null output buffer");
-            hipSetDevice(s->device);
+            check(hipSetDevice(s->device), "hipSetDevice");
             s->copy_sync(out, s->dt_log, static_cast<size_t>(n) * sizeof(double),
                          hipMemcpyDeviceToHost, "hipMemcpy(dt_log)");
         }

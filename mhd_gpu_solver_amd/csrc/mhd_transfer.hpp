@@ -169,8 +169,8 @@ public:
     void release()
     {
         for (int b = 0; b < 2; ++b) {
-            if (buf_[b]) hipHostFree(buf_[b]);
-            if (done_[b]) hipEventDestroy(done_[b]);
+            if (buf_[b]) (void)hipHostFree(buf_[b]);
+            if (done_[b]) (void)hipEventDestroy(done_[b]);
             buf_[b] = nullptr;
             done_[b] = nullptr;
         }
