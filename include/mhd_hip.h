@@ -152,8 +152,9 @@ int mhd_hip_last_solve_seconds(const mhd_hip_solver* s, double* seconds);
 /* Rows of the grid marched by one wavefront of the stage kernels (the tiling's only free
  * parameter). 0 (default) selects the autotuned value: on the first solve of a shape the pair
  * of stage kernels of one step is timed for candidate lengths and the fastest is kept for the
- * process. Results do not depend on this value. mhd_hip_get_seg_rows reports the value last
- * used. */
+ * process; a positive integer in the environment variable MHD_HIP_SEG_ROWS replaces the
+ * tuning for every shape. Results do not depend on this value. mhd_hip_get_seg_rows reports
+ * the value last used. */
 int mhd_hip_set_seg_rows(mhd_hip_solver* s, int rows);
 int mhd_hip_get_seg_rows(mhd_hip_solver* s, int* rows);
 
