@@ -1,8 +1,10 @@
-"""Summaries of rocprofv3 CSV output for the hip backend's kernels.
+r"""Summaries of rocprofv3 CSV output for the hip backend's kernels.
 
     python -m mhd_gpu_solver_amd.profiling trace  <kernel_trace.csv> [last N]
     python -m mhd_gpu_solver_amd.profiling traffic <fetch counter_collection.csv> \
         <write counter_collection.csv> <copy bytes> <interior cells per launch> [kernel filter]
+    python -m mhd_gpu_solver_amd.profiling valu <SQ counter_collection.csv> <kernel_trace.csv> \
+        <SIMDs> <clock GHz> <last N> [kernel filter]
 
 ``trace`` prints per-kernel launch statistics. Launches shorter than ``NOOP_US`` are the
 early-exit steps at the end of a fixed-size batch and are counted apart from the active ones.
@@ -10,6 +12,12 @@ With ``last N`` it adds the mean of each kernel's last N active launches: the ti
 bench.py leg come last, after its autotuning and warm-up launches.
 ``traffic`` converts FETCH_SIZE / WRITE_SIZE (KiB per dispatch) into bytes per launch, scaled
 by what the same counter reads for a plain device copy of known size in the same run.
+``valu`` takes SQ_INSTS_VALU / SQ_INSTS_SALU (wave instructions per dispatch, the mean of each
+kernel's last N active launches) from a counter run and the mean duration of the same kernels'
+last N active launches from a trace run without counters, and gives the VALU issue
+utilisation: instructions x 4 cycles (one wave64 instruction on a 16-lane SIMD) over SIMDs x
+clock x duration. Quarter-rate instructions (transcendentals, 32-bit integer multiplies) hold
+a SIMD for 16 cycles, so the figure is a lower bound on the SIMDs' busy time.
 """
 from __future__ import annotations
 
@@ -95,6 +103,41 @@ def traffic(fetch_csv: str, write_csv: str, copy_bytes: int, cells: int,
     return res
 
 
+def _last_active(values, last):
+    act = [v for v in values if v[1] > NOOP_US]
+    return act[-last:] if last else act
+
+
+def valu(counter_csv: str, trace_csv: str, simds: int, clock_ghz: float, last: int,
+         name_filter: str = "stage") -> str:
+    per = collections.defaultdict(lambda: collections.defaultdict(list))
+    rows = sorted(csv.DictReader(open(counter_csv)), key=lambda r: int(r["Start_Timestamp"]))
+    for r in rows:
+        k = _short(r["Kernel_Name"])
+        if name_filter not in k:
+            continue
+        d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+        per[k][r["Counter_Name"]].append((float(r["Counter_Value"]), d))
+    durs = collections.defaultdict(list)
+    for r in sorted(csv.DictReader(open(trace_csv)), key=lambda r: int(r["Start_Timestamp"])):
+        d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+        durs[_short(r["Kernel_Name"])].append((0.0, d))
+    out = ["| kernel | VALU instructions per launch | SALU per launch | mean duration (us) | "
+           "VALU issue utilisation |", "|---|---|---|---|---|"]
+    for k in sorted(per):
+        v = _last_active(per[k].get("SQ_INSTS_VALU", []), last)
+        sa = _last_active(per[k].get("SQ_INSTS_SALU", []), last)
+        dd = _last_active(durs.get(k, []), last)
+        if not v or not dd:
+            continue
+        nv = sum(x for x, _ in v) / len(v)
+        ns = sum(x for x, _ in sa) / len(sa) if sa else float("nan")
+        dur = sum(d for _, d in dd) / len(dd)
+        util = nv * 4.0 / (simds * clock_ghz * 1e3 * dur)
+        out.append(f"| {k} | {nv / 1e6:.1f} M | {ns / 1e6:.1f} M | {dur:.1f} | {util * 100:.1f} % |")
+    return "\n".join(out)
+
+
 def main(argv):
     if len(argv) >= 2 and argv[0] == "trace":
         last = int(argv[3]) if len(argv) >= 4 and argv[2] == "last" else 0
@@ -102,6 +145,9 @@ def main(argv):
     elif len(argv) >= 5 and argv[0] == "traffic":
         filt = argv[5] if len(argv) >= 6 else "stage"
         print(json.dumps(traffic(argv[1], argv[2], int(argv[3]), int(argv[4]), filt), indent=1))
+    elif len(argv) >= 6 and argv[0] == "valu":
+        filt = argv[6] if len(argv) >= 7 else "stage"
+        print(valu(argv[1], argv[2], int(argv[3]), float(argv[4]), int(argv[5]), filt))
     else:
         print(__doc__)
         return 2
