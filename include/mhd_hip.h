@@ -145,6 +145,15 @@ int mhd_hip_totals(mhd_hip_solver* s, double sums[9]);
  * (solver_api.hpp:344-347, inititial_condition.hpp:273-287) for the same state. */
 int mhd_hip_max_div_b(mhd_hip_solver* s, double* value);
 
+/* Relative L1 distance of one device state from another of the same grid, per conserved
+ * component over the interior, computed on the device: l1[c] = sum |a - ref| / sum |ref| (the
+ * absolute sum where sum |ref| is 0). The states may differ in precision (FP32 against fp64);
+ * an oracle state is compared by uploading it into a second solver. Double-double sums in a
+ * fixed order: the same bits on every call. Both solvers must be on one device.
+ * No counterpart call in the reference, which compares downloaded states on the host
+ * (tools/mhd_backend_compare.cpp:169-207); the per-field relative L1 is its accuracy metric. */
+int mhd_hip_rel_l1(mhd_hip_solver* a, mhd_hip_solver* ref, double l1[9]);
+
 /* Seconds of device time spent in the last solve/advance call (HIP events on the solver's
  * stream around the step loop, excluding upload/download). */
 int mhd_hip_last_solve_seconds(const mhd_hip_solver* s, double* seconds);
@@ -275,6 +284,11 @@ int mhd_hip_batch_max_div_b(mhd_hip_batch* b, int id, double* value);
 
 /* mhd_hip_totals for one instance (interior, double-double, deterministic). */
 int mhd_hip_batch_totals(mhd_hip_batch* b, int id, double sums[9]);
+
+/* mhd_hip_rel_l1 for instance id_a of engine a against instance id_ref of engine ref (the
+ * same engine or another on the same device, e.g. FP32 against fp64). */
+int mhd_hip_batch_rel_l1(mhd_hip_batch* a, int id_a, mhd_hip_batch* ref, int id_ref,
+                         double l1[9]);
 
 /* Rows per wave for the batch kernel. 0 (default): chosen for the current stepping set,
  * whenever it changes, by replaying the kernel's round-robin tile assignment and minimising

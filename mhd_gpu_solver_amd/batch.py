@@ -100,6 +100,15 @@ class BatchEngine:
         _lib.check(self._l.mhd_hip_batch_totals(self.handle, int(iid), out))
         return np.array(out[:])
 
+    def rel_l1(self, iid: int, ref: "BatchEngine", ref_iid: int):
+        """Relative L1 distance per conserved component of instance `iid` from instance
+        `ref_iid` of engine `ref` (this engine or another, e.g. fp64 against FP32), over the
+        interior, computed on the device (DeviceSolver.rel_l1)."""
+        out = (C.c_double * 9)()
+        _lib.check(self._l.mhd_hip_batch_rel_l1(self.handle, int(iid), ref.handle, int(ref_iid),
+                                                out))
+        return np.array(out[:])
+
     def max_div_b(self, iid: int) -> float:
         """max |div B| of one instance on the device (after its final ghost fill)."""
         v = C.c_double()

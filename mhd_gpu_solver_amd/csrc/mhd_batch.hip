@@ -154,6 +154,7 @@ struct mhd_hip_batch {
 
     mhdhip::DD* totals_partial = nullptr;
     double* totals_out = nullptr;
+    mhdhip::DD* l1_partial = nullptr;  // state-comparison scratch, allocated on first use
 
     bool dirty = true;
     int sched_policy = MHD_HIP_SCHED_ALL;
@@ -190,6 +191,7 @@ struct mhd_hip_batch {
         if (h_stage) (void)hipHostFree(h_stage);
         if (totals_partial) (void)hipFree(totals_partial);
         if (totals_out) (void)hipFree(totals_out);
+        if (l1_partial) (void)hipFree(l1_partial);
         for (hipEvent_t e : tev) (void)hipEventDestroy(e);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -823,6 +825,41 @@ struct mhd_hip_batch {
         mhdhip::scatter_planes(comps, reinterpret_cast<const R*>(h_stage), 9, n);
     }
 
+    // Relative L1 distance per component of instance `id` from instance `ref_id` of `ref`
+    // (the reference; possibly this engine), over the interior, on this engine's stream after
+    // `ref`'s pending work.
+    void rel_l1(int id, mhd_hip_batch& ref, int ref_id, double out[9])
+    {
+        const Slot& s = slot_of(id);
+        const Slot& r = ref.slot_of(ref_id);
+        if (s.desc.nx != r.desc.nx || s.desc.ny != r.desc.ny) {
+            throw InvalidArgument("rel_l1: the two instances must have the same grid");
+        }
+        if (device != ref.device) {
+            throw InvalidArgument("rel_l1: the two engines are on different devices");
+        }
+        check(hipSetDevice(device), "hipSetDevice");
+        if (!l1_partial) {
+            check(hipMalloc(reinterpret_cast<void**>(&l1_partial),
+                            sizeof(mhdhip::DD) * 18 * mhdhip::kTotalsBlocks),
+                  "hipMalloc(rel_l1)");
+        }
+        if (!totals_out) {
+            check(hipMalloc(reinterpret_cast<void**>(&totals_out), 9 * sizeof(double)),
+                  "hipMalloc(totals)");
+        }
+        check(hipStreamSynchronize(ref.stream), "hipStreamSynchronize");
+        const void* a = static_cast<const char*>(arena_base) + s.off;
+        const void* b = static_cast<const char*>(ref.arena_base) + r.off;
+        check(mhdhip::launch_rel_l1_any(a, precision == MHD_HIP_FP64, b,
+                                        ref.precision == MHD_HIP_FP64, geom_of(s.desc),
+                                        l1_partial, totals_out, stream),
+              "rel_l1");
+        check(hipMemcpyAsync(out, totals_out, 9 * sizeof(double), hipMemcpyDeviceToHost, stream),
+              "hipMemcpyAsync(rel_l1)");
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    }
+
     void totals(int id, double out[9])
     {
         const Slot& s = slot_of(id);
@@ -974,6 +1011,15 @@ int mhd_hip_batch_download(mhd_hip_batch* b, int id, double* const comps[9], siz
             if (!comps[c]) throw InvalidArgument("null component pointer");
         }
         b->download(id, comps, n);
+    });
+}
+
+int mhd_hip_batch_rel_l1(mhd_hip_batch* a, int id_a, mhd_hip_batch* ref, int id_ref,
+                         double l1[9])
+{
+    return guarded([&] {
+        if (!a || !ref || !l1) throw InvalidArgument("null argument");
+        a->rel_l1(id_a, *ref, id_ref, l1);
     });
 }
 

@@ -475,6 +475,87 @@ __global__ __launch_bounds__(kNC * kWave) void k_totals_final(const DD* __restri
     }
 }
 
+// Relative L1 distance of state A from a reference state B of the same grid, per component,
+// over the interior cells: per block, double-double sums of |a - b| and of |b| (partial[b][c]
+// and partial[b][kNC + c]); the final kernel gives sum |a - b| / sum |b| (the absolute sum
+// where sum |b| is 0). The difference is taken in fp64 from the stored values, so states of
+// either precision compare exactly as stored. Fixed-order reductions: the same bits on every
+// call.
+template <typename RA, typename RB>
+__global__ __launch_bounds__(kTotalsBlock) void k_l1_partial(const RA* __restrict__ A,
+                                                             const RB* __restrict__ B, Geom g,
+                                                             DD* __restrict__ partial)
+{
+    __shared__ DD red[2 * kNC][kTotalsBlock / kWave];
+    const long long ncell = static_cast<long long>(g.nx) * g.ny;
+    DD diff[kNC], ref[kNC];
+#pragma unroll
+    for (int c = 0; c < kNC; ++c) {
+        diff[c] = DD{0.0, 0.0};
+        ref[c] = DD{0.0, 0.0};
+    }
+    for (long long k = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; k < ncell;
+         k += static_cast<long long>(gridDim.x) * blockDim.x) {
+        const int jj = static_cast<int>(k / g.nx);
+        const int ii = static_cast<int>(k - static_cast<long long>(jj) * g.nx);
+        const long long cell = static_cast<long long>(jj + g.ng) * g.nxt + (ii + g.ng);
+#pragma unroll
+        for (int c = 0; c < kNC; ++c) {
+            const double a = static_cast<double>(A[c * g.plane + cell]);
+            const double b = static_cast<double>(B[c * g.plane + cell]);
+            diff[c] = dd_add(diff[c], fabs(a - b));
+            ref[c] = dd_add(ref[c], fabs(b));
+        }
+    }
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < 2 * kNC; ++c) {
+        DD v = c < kNC ? diff[c] : ref[c - kNC];
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const DD o{__shfl_down(v.hi, off), __shfl_down(v.lo, off)};
+            v = dd_add(v, o);
+        }
+        if (lane == 0) {
+            red[c][wave] = v;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * kNC) {
+        DD v = red[threadIdx.x][0];
+        for (int w = 1; w < kTotalsBlock / kWave; ++w) {
+            v = dd_add(v, red[threadIdx.x][w]);
+        }
+        partial[blockIdx.x * 2 * kNC + threadIdx.x] = v;
+    }
+}
+
+template <int NC>  // kNC; a template so that every translation unit may include it
+__global__ __launch_bounds__(NC * kWave) void k_l1_final(const DD* __restrict__ partial,
+                                                         int blocks, double* __restrict__ out)
+{
+    const int c = threadIdx.x >> 6;
+    const int lane = threadIdx.x & (kWave - 1);
+    DD d{0.0, 0.0}, r{0.0, 0.0};
+    for (int b = lane; b < blocks; b += kWave) {
+        d = dd_add(d, partial[b * 2 * NC + c]);
+        r = dd_add(r, partial[b * 2 * NC + NC + c]);
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const DD od{__shfl_down(d.hi, off), __shfl_down(d.lo, off)};
+        const DD orr{__shfl_down(r.hi, off), __shfl_down(r.lo, off)};
+        d = dd_add(d, od);
+        r = dd_add(r, orr);
+    }
+    if (lane == 0) {
+        const double sd = d.hi + d.lo;
+        const double sr = r.hi + r.lo;
+        out[c] = sr > 0.0 ? sd / sr : sd;
+    }
+}
+
 // ---- fused RK stage ------------------------------------------------------------------------
 //
 // One wave owns a strip of 60 interior columns and marches rows [r0, r1). Lane l holds global

@@ -37,6 +37,33 @@ constexpr int kTotalsBlocks = 512;
 template <typename R>
 hipError_t launch_totals(const R* U, const Geom& g, DD* partial, double* out, hipStream_t s);
 
+// Relative L1 distance per component of A from the reference B (same grid) over the interior:
+// `partial` must hold kTotalsBlocks * 18 DD values, `out` 9 doubles. Instantiated for every
+// pairing of float and double.
+template <typename RA, typename RB>
+hipError_t launch_rel_l1(const RA* A, const RB* B, const Geom& g, DD* partial, double* out,
+                         hipStream_t s);
+
+// The same on type-erased states: a64 / b64 tell whether A / B store fp64 (else fp32).
+inline hipError_t launch_rel_l1_any(const void* A, bool a64, const void* B, bool b64,
+                                    const Geom& g, DD* partial, double* out, hipStream_t s)
+{
+    if (a64 && b64) {
+        return launch_rel_l1(static_cast<const double*>(A), static_cast<const double*>(B), g,
+                             partial, out, s);
+    }
+    if (a64) {
+        return launch_rel_l1(static_cast<const double*>(A), static_cast<const float*>(B), g,
+                             partial, out, s);
+    }
+    if (b64) {
+        return launch_rel_l1(static_cast<const float*>(A), static_cast<const double*>(B), g,
+                             partial, out, s);
+    }
+    return launch_rel_l1(static_cast<const float*>(A), static_cast<const float*>(B), g, partial,
+                         out, s);
+}
+
 // max |div B| of the interior by central differences (stored ghosts used) into *out.
 template <typename R>
 hipError_t launch_max_div_b(const R* U, const Geom& g, double inv_2dx, double inv_2dy,

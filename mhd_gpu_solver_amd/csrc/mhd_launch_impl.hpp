@@ -175,6 +175,20 @@ hipError_t launch_totals(const R* U, const Geom& g, DD* partial, double* out, hi
     return hipGetLastError();
 }
 
+template <typename RA, typename RB>
+hipError_t launch_rel_l1(const RA* A, const RB* B, const Geom& g, DD* partial, double* out,
+                         hipStream_t s)
+{
+    const long long n = static_cast<long long>(g.nx) * g.ny;
+    const int blocks = static_cast<int>(std::max<long long>(
+        1, std::min<long long>(kTotalsBlocks, (n + kTotalsBlock - 1) / kTotalsBlock)));
+    hipLaunchKernelGGL((k_l1_partial<RA, RB>), dim3(blocks), dim3(kTotalsBlock), 0, s, A, B, g,
+                       partial);
+    hipLaunchKernelGGL(k_l1_final<kNC>, dim3(1), dim3(kNC * kWave), 0, s, partial, blocks,
+                       out);
+    return hipGetLastError();
+}
+
 template <typename R>
 int stage_blocks_per_cu(int limiter, bool nonideal, bool remap)
 {

@@ -119,6 +119,7 @@ struct mhd_hip_solver {
     double* totals_out = nullptr;
 
     mhdhip::StagedCopier staging;  // pinned chunk buffers for large uploads and downloads
+    mhdhip::DD* l1_partial = nullptr;  // state-comparison scratch, allocated on first use
 
     ~mhd_hip_solver() { release(); }
 
@@ -137,8 +138,10 @@ struct mhd_hip_solver {
         if (plan_dev) (void)hipFree(plan_dev);
         if (totals_partial) (void)hipFree(totals_partial);
         if (totals_out) (void)hipFree(totals_out);
+        if (l1_partial) (void)hipFree(l1_partial);
         totals_partial = nullptr;
         totals_out = nullptr;
+        l1_partial = nullptr;
         staging.release();
         for (hipEvent_t e : timing_events) (void)hipEventDestroy(e);
         timing_events.clear();
@@ -783,6 +786,30 @@ struct mhd_hip_solver {
         check(hipStreamSynchronize(stream), "hipStreamSynchronize");
     }
 
+    // Relative L1 distance per component of this solver's device state from `ref`'s (the
+    // reference), over the interior, on this solver's stream after `ref`'s pending work.
+    void rel_l1(mhd_hip_solver& ref, double out[9])
+    {
+        check(hipSetDevice(device), "hipSetDevice");
+        if (!l1_partial) {
+            check(hipMalloc(reinterpret_cast<void**>(&l1_partial),
+                            sizeof(mhdhip::DD) * 18 * mhdhip::kTotalsBlocks),
+                  "hipMalloc(rel_l1)");
+        }
+        if (!totals_out) {
+            check(hipMalloc(reinterpret_cast<void**>(&totals_out), 9 * sizeof(double)),
+                  "hipMalloc(totals)");
+        }
+        check(hipStreamSynchronize(ref.stream), "hipStreamSynchronize");
+        check(mhdhip::launch_rel_l1_any(U, precision == MHD_HIP_FP64, ref.U,
+                                        ref.precision == MHD_HIP_FP64, geom, l1_partial,
+                                        totals_out, stream),
+              "rel_l1");
+        check(hipMemcpyAsync(out, totals_out, 9 * sizeof(double), hipMemcpyDeviceToHost, stream),
+              "hipMemcpyAsync(rel_l1)");
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    }
+
     double max_div_b()
     {
         check(hipSetDevice(device), "hipSetDevice");
@@ -1007,6 +1034,21 @@ int mhd_hip_totals(mhd_hip_solver* s, double sums[9])
     return guarded([&] {
         if (!s || !sums) throw InvalidArgument("null argument");
         s->totals(sums);
+    });
+}
+
+int mhd_hip_rel_l1(mhd_hip_solver* a, mhd_hip_solver* ref, double l1[9])
+{
+    return guarded([&] {
+        if (!a || !ref || !l1) throw InvalidArgument("null argument");
+        if (a->grid.nx != ref->grid.nx || a->grid.ny != ref->grid.ny ||
+            a->grid.ng != ref->grid.ng) {
+            throw InvalidArgument("rel_l1: the two states must have the same grid");
+        }
+        if (a->device != ref->device) {
+            throw InvalidArgument("rel_l1: the two solvers are on different devices");
+        }
+        a->rel_l1(*ref, l1);
     });
 }
 

@@ -267,6 +267,14 @@ class DeviceSolver:
         _lib.check(self._l.mhd_hip_max_div_b(self.handle, C.byref(v)))
         return v.value
 
+    def rel_l1(self, ref: "DeviceSolver"):
+        """Relative L1 distance per conserved component of this device state from `ref`'s
+        (same grid, either precision), over the interior, computed on the device:
+        sum |a - ref| / sum |ref| (the absolute sum where sum |ref| is 0)."""
+        out = (C.c_double * 9)()
+        _lib.check(self._l.mhd_hip_rel_l1(self.handle, ref.handle, out))
+        return np.array(out[:])
+
     def set_seg_rows(self, rows: int):
         """Rows marched per wavefront (0 = autotuned); results do not depend on it."""
         _lib.check(self._l.mhd_hip_set_seg_rows(self.handle, int(rows)))

@@ -91,3 +91,63 @@ def test_batch_max_div_b_equals_oracle(hip, oracle):
         vals.append(eng.max_div_b(i))
         assert vals[-1] == oracle.max_div_b(make_grid(sp.n, sp.n), eng.download(i))
     assert max(vals) > 0.0
+
+
+def rel_l1_exact(A, B):
+    """sum |a - b| / sum |b| over the interior per component (the absolute sum where sum |b| is
+    0), with the differences taken in fp64 from the stored values and exactly rounded sums."""
+    out = []
+    for c in range(9):
+        a = A[c, 2:-2, 2:-2].ravel()
+        b = B[c, 2:-2, 2:-2].ravel()
+        d = math.fsum(np.abs(a - b).tolist())
+        r = math.fsum(np.abs(b).tolist())
+        out.append(d / r if r > 0 else d)
+    return np.array(out)
+
+
+def test_rel_l1_on_device_fp32_against_fp64(hip):
+    """The device comparison of an FP32 state with an fp64 state of the same solve equals the
+    exactly rounded host computation to double-double accuracy, is zero for a state against
+    itself, gives the same bits on every call and refuses states of different grids."""
+    states = {}
+    for prec in ("fp32", "fp64"):
+        s = hip.MHDSolver(173, 91)
+        s.set_backend("hip", prec)
+        s.initialize("rotor")
+        s.run(0.01)
+        dev = s._dev()
+        dev.upload(s.state)          # the state the host holds, exactly representable
+        states[prec] = (s, dev)
+    (s32, d32), (s64, d64) = states["fp32"], states["fp64"]
+    got = d32.rel_l1(d64)
+    want = rel_l1_exact(s32.state, s64.state)
+    assert np.allclose(got, want, rtol=1e-13, atol=0.0), (got, want)
+    assert max(got[:8]) <= 1e-4
+    assert np.array_equal(d32.rel_l1(d64), got)
+    assert np.array_equal(d64.rel_l1(d64), np.zeros(9))
+    other = hip.MHDSolver(172, 91)
+    other.set_backend("hip", "fp64")
+    other.initialize("rotor")
+    with pytest.raises(ValueError):
+        d64.rel_l1(other._dev())
+
+
+def test_batch_rel_l1_across_engines(hip):
+    """Instance against instance across an FP32 and an fp64 engine, on the device, equals the
+    host computation from the downloaded states."""
+    from mhd_gpu_solver_amd import workloads
+    spec = workloads.Spec("orszag_tang_b0", 130, 1.5, 1.2, (0, 0), t_end=0.02)
+    U0 = spec.state()
+    engines, ids = {}, {}
+    for prec in ("fp32", "fp64"):
+        eng = hip.BatchEngine(prec, limiter=hip.SlopeLimiter.MC, hbm_budget=1 << 28)
+        ids[prec] = eng.add(U0, spec.t_end, gamma=spec.gamma, bc=spec.bc)
+        eng.advance()
+        engines[prec] = eng
+    got = engines["fp32"].rel_l1(ids["fp32"], engines["fp64"], ids["fp64"])
+    want = rel_l1_exact(engines["fp32"].download(ids["fp32"]),
+                        engines["fp64"].download(ids["fp64"]))
+    assert np.allclose(got, want, rtol=1e-13, atol=0.0), (got, want)
+    assert np.array_equal(engines["fp64"].rel_l1(ids["fp64"], engines["fp64"], ids["fp64"]),
+                          np.zeros(9))
