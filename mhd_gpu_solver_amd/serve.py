@@ -127,6 +127,13 @@ class Server:
         max_conc = 0
         replays = 0
         advance_s = 0.0
+
+        def prepare_window():
+            # the first `prepare_ahead` queued requests have their states under way
+            for k in itertools.islice(queue, prepare_ahead):
+                if k not in states:
+                    states[k] = prep.submit(requests[k].state)
+
         while pending or queue or active:
             now = clock() - t0
             while pending and (not realtime or requests[pending[0]].arrival <= now):
@@ -135,16 +142,16 @@ class Server:
                 if instance_bytes(self.precision, req.n, req.n) > self.hbm_budget:
                     raise ValueError(f"request {k} ({req.n}^2) exceeds the whole HBM budget")
                 queue.append(k)
-            for k in itertools.islice(queue, prepare_ahead):
-                if k not in states:
-                    states[k] = prep.submit(requests[k].state)
+            prepare_window()
             while queue:
                 k = queue[0]
                 req = requests[k]
-                if active and not states[k].done():
+                # the head is always inside the window, whose edge moves with every admission
+                fut = states[k]
+                if active and not fut.done():
                     break  # the device has work; admit this one after the next replay
                 try:
-                    iid = self.engine.add(states[k].result(), req.t_end, cfl=self.cfl,
+                    iid = self.engine.add(fut.result(), req.t_end, cfl=self.cfl,
                                           gamma=req.gamma, glm_alpha=self.glm_alpha, bc=req.bc,
                                           max_steps=req.max_steps)
                 except CapacityFull:
@@ -152,6 +159,7 @@ class Server:
                 queue.popleft()
                 del states[k]
                 active[iid] = (k, clock() - t0)
+                prepare_window()
             max_conc = max(max_conc, len(active))
             if not active:
                 if pending and realtime:

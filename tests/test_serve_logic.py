@@ -104,6 +104,49 @@ def test_all_complete_fifo_and_states_off_the_loop():
     assert rep.advance_s <= rep.makespan
 
 
+class InlineExecutor:
+    """Builds each state at submission, on the loop's thread: every future is already done
+    when the loop looks at it, so admission runs ahead of the preparation window at once."""
+
+    def __init__(self, max_workers=None):
+        pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        return False
+
+    def submit(self, fn, *args):
+        from concurrent.futures import Future
+        f = Future()
+        f.set_result(fn(*args))
+        return f
+
+
+def test_admission_past_the_preparation_window(monkeypatch):
+    """More admissible requests than `prepare_ahead`, every state ready at once: the loop
+    admits past the first window (whose edge moves with each admission) in FIFO order."""
+    monkeypatch.setattr(serve, "ThreadPoolExecutor", InlineExecutor)
+    eng = FakeEngine()
+    reqs = [Req(k, replays=1 + k % 3) for k in range(9)]
+    rep = make_server(eng).run(reqs, realtime=False, prepare_workers=1, prepare_ahead=2)
+    assert eng.order == list(range(9))
+    assert eng.added_at == [0] * 9                      # all admitted before the first replay
+    assert sorted(c.index for c in rep.completed) == list(range(9))
+
+
+def test_admission_window_with_threads_repeated():
+    """The same with real preparation threads and no delays, whatever order the futures
+    finish in: every request completes once, FIFO."""
+    for _ in range(20):
+        eng = FakeEngine()
+        reqs = [Req(k, replays=1 + k % 4) for k in range(12)]
+        rep = make_server(eng).run(reqs, realtime=False, prepare_workers=3, prepare_ahead=2)
+        assert eng.order == list(range(12))
+        assert sorted(c.index for c in rep.completed) == list(range(12))
+
+
 def test_slow_state_waits_while_others_run():
     """The head's state takes long; the loop keeps replaying the running solves meanwhile,
     and every request still completes in FIFO order."""
