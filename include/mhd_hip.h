@@ -222,7 +222,7 @@ typedef struct mhd_hip_batch_stats {
     int instances; /* admitted (running or finished, not yet removed) */
     int active;    /* still stepping (unfinished, not paused) */
     int tiles;     /* wave work items in the current tile list */
-    int seg_rows;  /* rows per wave of the current tile list (load model unless set) */
+    int seg_rows;  /* fixed rows per tile of the current tile list; 0: capacity fill */
     int blocks;    /* workgroups of the persistent stage kernel */
     size_t bytes_used, bytes_budget;
     double last_seconds; /* device time of the last mhd_hip_batch_advance */
@@ -247,7 +247,9 @@ int mhd_hip_batch_add(mhd_hip_batch* b, const mhd_hip_instance* inst, const doub
 int mhd_hip_batch_remove(mhd_hip_batch* b, int id);
 
 /* Replays the step graph until no instance is active, or at most max_replays times
- * (max_replays < 0: no limit). *active receives the number of instances still stepping. */
+ * (max_replays < 0: no limit). *active receives the number of instances still stepping.
+ * MHD_HIP_ERUNTIME if a replay advanced none of the instances it stepped (a replay that makes
+ * no progress would repeat forever). */
 int mhd_hip_batch_advance(mhd_hip_batch* b, int max_replays, int* active);
 
 /* Result so far (t, steps, c_h as SolveResult) and whether the instance has finished; the
@@ -290,11 +292,13 @@ int mhd_hip_batch_totals(mhd_hip_batch* b, int id, double sums[9]);
 int mhd_hip_batch_rel_l1(mhd_hip_batch* a, int id_a, mhd_hip_batch* ref, int id_ref,
                          double l1[9]);
 
-/* Rows per wave for the batch kernel. 0 (default): chosen for the current stepping set,
- * whenever it changes, by replaying the kernel's round-robin tile assignment and minimising
- * the busiest wave's rows (suits a set that changes, as in serving); -1: one stage-1 pass per
- * candidate is timed once, on the first advance's set (the better choice for a set that stays
- * the same); > 0: fixed. Results do not depend on this value. */
+/* Work partition of the batch kernel. Each wave of the persistent grid marches its own list
+ * of tiles (instance, strip, row range). 0 (default): capacity fill, re-planned whenever the
+ * stepping set changes: every strip is cut into row pieces poured into the waves in order up
+ * to a common capacity, so all waves carry about the same work; -1: the fill and fixed row
+ * lengths are each timed once (one stage-1 pass) on the first advance's set and the fastest
+ * is kept; > 0: fixed segments of that many rows handed out round-robin. Results do not depend
+ * on this value. */
 int mhd_hip_batch_set_seg_rows(mhd_hip_batch* b, int rows);
 
 int mhd_hip_batch_info(const mhd_hip_batch* b, mhd_hip_batch_stats* stats);

@@ -8,6 +8,7 @@
 // join (admission, first CFL reduction) and leave between replays by copies on the engine's
 // stream, so the captured graph stays valid across changes of the instance set.
 #include "mhd_abi.hpp"
+#include "mhd_batch_plan.hpp"
 #include "mhd_hip.h"
 #include "mhd_launch.hpp"
 #include "mhd_transfer.hpp"
@@ -141,13 +142,13 @@ struct mhd_hip_batch {
     BatchInst* d_insts = nullptr;
     Ctrl* d_ctrl = nullptr;
     BatchTile* d_tiles = nullptr;
-    int* d_ntiles = nullptr;
+    int* d_first = nullptr;  // per-wave tile offsets (waves + 1)
     size_t tile_cap = 0;
 
     BatchInst* h_insts = nullptr;  // pinned staging
     Ctrl* h_ctrl = nullptr;
     BatchTile* h_tiles = nullptr;
-    int* h_ntiles = nullptr;
+    int* h_first = nullptr;
     size_t h_tile_cap = 0;
     double* h_stage = nullptr;  // pinned staging for state upload/download
     size_t h_stage_bytes = 0;
@@ -159,9 +160,11 @@ struct mhd_hip_batch {
     bool dirty = true;
     int sched_policy = MHD_HIP_SCHED_ALL;
     long long sched_cap = 0;  // interior cells stepping per replay under SRPT
-    int seg_rows = 0;  // > 0 fixed; 0 load model at every table build; -1 timed once
-    int tuned_rows = 0;
-    int last_rows = 0;  // rows per wave of the current tile list
+    // Work partition: > 0 fixed segments of that many rows, handed out round-robin; 0 the
+    // capacity fill (plan_fill) at every table build; -1 timed once over the candidates.
+    int seg_rows = 0;
+    int tuned_rows = 0;  // the timed choice: > 0 fixed rows, < 0 fill with -tuned_rows min rows
+    int last_rows = 0;  // fixed rows of the current tile list (0: capacity fill)
     int ntiles = 0;
     int blocks = 0;
     double last_seconds = 0.0;
@@ -183,11 +186,11 @@ struct mhd_hip_batch {
         if (d_insts) (void)hipFree(d_insts);
         if (d_ctrl) (void)hipFree(d_ctrl);
         if (d_tiles) (void)hipFree(d_tiles);
-        if (d_ntiles) (void)hipFree(d_ntiles);
+        if (d_first) (void)hipFree(d_first);
         if (h_insts) (void)hipHostFree(h_insts);
         if (h_ctrl) (void)hipHostFree(h_ctrl);
         if (h_tiles) (void)hipHostFree(h_tiles);
-        if (h_ntiles) (void)hipHostFree(h_ntiles);
+        if (h_first) (void)hipHostFree(h_first);
         if (h_stage) (void)hipHostFree(h_stage);
         if (totals_partial) (void)hipFree(totals_partial);
         if (totals_out) (void)hipFree(totals_out);
@@ -199,10 +202,19 @@ struct mhd_hip_batch {
     }
 
     // One replay's worth of steps, launched eagerly with an event pair around each stage
-    // kernel; the kernel times are added after the next synchronisation.
+    // kernel. After the synchronisation the control blocks tell how many of the kSteps steps
+    // stepped some instance (the later launches found every instance finished and returned at
+    // once): only those launches and their times are added.
     template <typename R>
     void timed_replay()
     {
+        std::vector<int> before(static_cast<size_t>(max_instances), 0);
+        std::vector<char> stepping(static_cast<size_t>(max_instances), 0);
+        for (int si = 0; si < max_instances; ++si) {
+            const Slot& s = slots[si];
+            stepping[si] = s.used && !s.finished && !s.paused;
+            before[si] = h_ctrl[si].steps[0];
+        }
         const size_t need = 4 * kSteps;
         while (tev.size() < need) {
             hipEvent_t e;
@@ -220,16 +232,24 @@ struct mhd_hip_batch {
                   "batch stage2");
             check(hipEventRecord(tev[4 * k + 3], stream), "hipEventRecord");
         }
+        check(hipMemcpyAsync(h_ctrl, d_ctrl, sizeof(Ctrl) * max_instances, hipMemcpyDeviceToHost,
+                             stream),
+              "hipMemcpyAsync(ctrl)");
         check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-        for (int k = 0; k < kSteps; ++k) {
+        std::vector<int> after(static_cast<size_t>(max_instances), 0);
+        for (int si = 0; si < max_instances; ++si) {
+            after[si] = h_ctrl[si].steps[0];
+        }
+        const int live = mhdhip::replay_active_launches(before, after, stepping, kSteps);
+        for (int k = 0; k < live; ++k) {
             float a = 0.0f, b = 0.0f;
             check(hipEventElapsedTime(&a, tev[4 * k], tev[4 * k + 1]), "hipEventElapsedTime");
             check(hipEventElapsedTime(&b, tev[4 * k + 2], tev[4 * k + 3]), "hipEventElapsedTime");
             stage_ms[0] += a;
             stage_ms[1] += b;
         }
-        stage_launches[0] += kSteps;
-        stage_launches[1] += kSteps;
+        stage_launches[0] += live;
+        stage_launches[1] += live;
     }
 
     void init(int prec, const mhd_hip_params& tmpl, size_t budget, int max_inst)
@@ -275,25 +295,29 @@ struct mhd_hip_batch {
               "hipMalloc(insts)");
         check(hipMalloc(reinterpret_cast<void**>(&d_ctrl), sizeof(Ctrl) * max_inst),
               "hipMalloc(ctrl)");
-        check(hipMalloc(reinterpret_cast<void**>(&d_ntiles), sizeof(int)), "hipMalloc(ntiles)");
-        check(hipHostMalloc(reinterpret_cast<void**>(&h_insts), sizeof(BatchInst) * max_inst, 0),
-              "hipHostMalloc");
-        check(hipHostMalloc(reinterpret_cast<void**>(&h_ctrl), sizeof(Ctrl) * max_inst, 0),
-              "hipHostMalloc");
-        check(hipHostMalloc(reinterpret_cast<void**>(&h_ntiles), sizeof(int), 0), "hipHostMalloc");
-        std::memset(h_insts, 0, sizeof(BatchInst) * max_inst);
-        std::memset(h_ctrl, 0, sizeof(Ctrl) * max_inst);
-        *h_ntiles = 0;
-        check(hipMemcpyAsync(d_insts, h_insts, sizeof(BatchInst) * max_inst,
-                             hipMemcpyHostToDevice, stream),
-              "hipMemcpyAsync");
-        check(hipMemcpyAsync(d_ntiles, h_ntiles, sizeof(int), hipMemcpyHostToDevice, stream),
-              "hipMemcpyAsync");
-        ensure_tiles(1024);
         const int per_cu = precision == MHD_HIP_FP64
                                ? mhdhip::batch_blocks_per_cu<double>(limiter, nonideal)
                                : mhdhip::batch_blocks_per_cu<float>(limiter, nonideal);
         blocks = std::max(1, per_cu) * prop.multiProcessorCount;
+        const size_t nfirst = static_cast<size_t>(waves()) + 1;
+        check(hipMalloc(reinterpret_cast<void**>(&d_first), sizeof(int) * nfirst),
+              "hipMalloc(first)");
+        check(hipHostMalloc(reinterpret_cast<void**>(&h_insts), sizeof(BatchInst) * max_inst, 0),
+              "hipHostMalloc");
+        check(hipHostMalloc(reinterpret_cast<void**>(&h_ctrl), sizeof(Ctrl) * max_inst, 0),
+              "hipHostMalloc");
+        check(hipHostMalloc(reinterpret_cast<void**>(&h_first), sizeof(int) * nfirst, 0),
+              "hipHostMalloc");
+        std::memset(h_insts, 0, sizeof(BatchInst) * max_inst);
+        std::memset(h_ctrl, 0, sizeof(Ctrl) * max_inst);
+        std::fill(h_first, h_first + nfirst, 0);
+        check(hipMemcpyAsync(d_insts, h_insts, sizeof(BatchInst) * max_inst,
+                             hipMemcpyHostToDevice, stream),
+              "hipMemcpyAsync");
+        check(hipMemcpyAsync(d_first, h_first, sizeof(int) * nfirst, hipMemcpyHostToDevice,
+                             stream),
+              "hipMemcpyAsync");
+        ensure_tiles(1024);
         check(hipStreamSynchronize(stream), "hipStreamSynchronize");
     }
 
@@ -319,11 +343,14 @@ struct mhd_hip_batch {
         dirty = true;
     }
 
+    // Pinned staging of at least `bytes`. A larger buffer replaces the old one only after the
+    // stream has drained, so no copy queued from or into the old buffer is still in flight.
     void ensure_stage(size_t bytes)
     {
         if (bytes <= h_stage_bytes) {
             return;
         }
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
         if (h_stage) (void)hipHostFree(h_stage);
         h_stage = nullptr;
         check(hipHostMalloc(reinterpret_cast<void**>(&h_stage), bytes, 0), "hipHostMalloc(stage)");
@@ -400,8 +427,8 @@ struct mhd_hip_batch {
     void upload_state(const Slot& s, const double* const comps[9])
     {
         const size_t n = state_elems(s.desc.nx, s.desc.ny);
-        ensure_stage(9 * n * sizeof(R));
         check(hipStreamSynchronize(stream), "hipStreamSynchronize");  // staging reuse
+        ensure_stage(9 * n * sizeof(R));
         R* st = reinterpret_cast<R*>(h_stage);
         mhdhip::gather_planes(st, comps, 9, n);
         char* base = static_cast<char*>(arena_base) + s.off;
@@ -496,93 +523,45 @@ struct mhd_hip_batch {
         dirty = true;
     }
 
-    // Row lengths the load model weighs. The timed mode also tries 2, 4 and 6: the model,
-    // which counts each wave's own work only, would pick those for sets that fit in one
-    // round of the grid, where the extra halo rows cost more than the shorter march saves.
-    static constexpr int kRowCandidates[] = {8, 12, 16, 20, 24, 28, 32, 40, 48, 64, 96, 128};
-    static constexpr int kTimedRowCandidates[] = {2,  4,  6,  8,  12, 16, 20, 24,
-                                                  28, 32, 40, 48, 64, 96, 128};
+    // Fixed row lengths the timed choice (rows per wave -1) tries beside the capacity fill
+    // with minimum pieces of kFillMinRows rows (plan_fill).
+    static constexpr int kTimedRowCandidates[] = {8, 16, 24, 32, 48, 64};
+    static constexpr int kFillMinRows[] = {2, 8};
 
-    // The tiles of one instance for `rows` rows per wave, in launch order: whole-wave tiles
-    // over its 60-column strips, row segment by row segment; then, when its last strip is at
-    // most kHalfCells columns wide, half-wave tiles for that strip, each pairing a chunk of the
-    // strip's lower half of rows (lanes 0..31) with the same chunk of its upper half
-    // (lanes 32..63). f(strip, r0, r1, r0b, r1b); r0b < 0 marks a whole-wave tile.
-    template <typename F>
-    static void for_each_tile(const mhd_hip_instance& d, int rows, F&& f)
+    int waves() const { return blocks * mhdhip::kWavesPerBlock; }
+
+    // Work partition of the stepping instances: `mode` > 0 fixed segments of that many rows
+    // handed out round-robin; mode <= 0 the capacity fill with minimum pieces of -mode rows
+    // (0: two rows).
+    mhdhip::WavePlan plan(int mode) const
     {
-        const int ng = 2;
-        const int full = d.nx / mhdhip::kStripCells;
-        const int tail = d.nx - full * mhdhip::kStripCells;
-        const bool half = tail > 0 && tail <= mhdhip::kHalfCells;
-        const int wide = half ? full : (d.nx + mhdhip::kStripCells - 1) / mhdhip::kStripCells;
-        for (int r0 = ng; r0 < ng + d.ny; r0 += rows) {
-            for (int st = 0; st < wide; ++st) {
-                f(st, r0, std::min(r0 + rows, ng + d.ny), -1, 0);
+        std::vector<mhdhip::PlanInst> insts;
+        for (int si = 0; si < max_instances; ++si) {
+            const Slot& s = slots[si];
+            if (s.used && !s.finished && !s.paused) {
+                insts.push_back(mhdhip::PlanInst{si, s.desc.nx, s.desc.ny});
             }
         }
-        if (half) {
-            const int h = (d.ny + 1) / 2;  // rows of the lower half (the upper has h or h - 1)
-            for (int a = 0; a < h; a += rows) {
-                const int r0 = ng + a;
-                const int r1 = ng + std::min(a + rows, h);
-                const int r0b = ng + h + a;
-                f(full, r0, r1, r0b, std::min(r0b + (r1 - r0), ng + d.ny));
-            }
+        if (mode > 0) {
+            return mhdhip::plan_fixed(insts, mode, waves());
         }
+        return mhdhip::plan_fill(insts, waves(), mode < 0 ? -mode : kFillMinRows[0]);
     }
 
-    // Rows per wave for the current load. The persistent kernel hands tile t to wave
-    // t mod (waves in the grid); a tile of L rows marches L + 2 rows and loads L + 3, so its
-    // cost is taken as 2 L + 5 half-rows. The model replays that assignment for each
-    // candidate length and keeps the one whose busiest wave has the least work, the longer
-    // length on ties (fewer halo re-reads).
-    int model_rows() const
-    {
-        const int waves = std::max(1, blocks * mhdhip::kWavesPerBlock);
-        std::vector<long long> load(static_cast<size_t>(waves));
-        int best = kRowCandidates[0];
-        long long best_cost = std::numeric_limits<long long>::max();
-        for (const int rows : kRowCandidates) {
-            std::fill(load.begin(), load.end(), 0ll);
-            long long k = 0;
-            for (const auto& s : slots) {
-                if (!s.used || s.finished || s.paused) {
-                    continue;
-                }
-                for_each_tile(s.desc, rows, [&](int, int r0, int r1, int, int) {
-                    load[static_cast<size_t>(k % waves)] += 2ll * (r1 - r0) + 5;
-                    ++k;
-                });
-            }
-            const long long cost = *std::max_element(load.begin(), load.end());
-            if (cost <= best_cost) {
-                best_cost = cost;
-                best = rows;
-            }
-        }
-        return best;
-    }
-
-    int rows_for_tiles() const
+    int plan_mode() const
     {
         if (seg_rows > 0) return seg_rows;
-        if (seg_rows < 0 && tuned_rows > 0) return tuned_rows;
-        return model_rows();
+        if (seg_rows < 0 && tuned_rows != 0) return tuned_rows;
+        return 0;
     }
 
-    void build_tables(int rows)
+    void build_tables(int mode)
     {
-        last_rows = rows;
-        size_t n = 0;
-        for (const auto& s : slots) {
-            if (s.used && !s.finished && !s.paused) {
-                for_each_tile(s.desc, rows, [&](int, int, int, int, int) { ++n; });
-            }
-        }
+        const mhdhip::WavePlan p = plan(mode);
+        last_rows = mode > 0 ? mode : 0;
+        const size_t n = p.tiles.size();
         ensure_tiles(std::max<size_t>(n, 1));
         check(hipStreamSynchronize(stream), "hipStreamSynchronize");  // staging reuse
-        size_t k = 0;
         for (int si = 0; si < max_instances; ++si) {
             const Slot& s = slots[si];
             BatchInst& in = h_insts[si];
@@ -596,28 +575,25 @@ struct mhd_hip_batch {
             in.g = geom_of(s.desc);
             in.ph = physics_of(s.desc);
             in.enabled = 1;
-            if (s.finished || s.paused) {
-                continue;
-            }
-            for_each_tile(s.desc, rows, [&](int st, int r0, int r1, int r0b, int r1b) {
-                h_tiles[k++] = BatchTile{si, st, r0, r1, r0b, r1b};
-            });
         }
-        ntiles = static_cast<int>(k);
-        *h_ntiles = ntiles;
+        std::copy(p.tiles.begin(), p.tiles.end(), h_tiles);
+        const size_t nfirst = static_cast<size_t>(waves()) + 1;
+        std::copy(p.first.begin(), p.first.end(), h_first);
+        ntiles = static_cast<int>(n);
         check(hipMemcpyAsync(d_insts, h_insts, sizeof(BatchInst) * max_instances,
                              hipMemcpyHostToDevice, stream),
               "hipMemcpyAsync(insts)");
-        if (k > 0) {
-            check(hipMemcpyAsync(d_tiles, h_tiles, sizeof(BatchTile) * k, hipMemcpyHostToDevice,
+        if (n > 0) {
+            check(hipMemcpyAsync(d_tiles, h_tiles, sizeof(BatchTile) * n, hipMemcpyHostToDevice,
                                  stream),
                   "hipMemcpyAsync(tiles)");
         }
-        check(hipMemcpyAsync(d_ntiles, h_ntiles, sizeof(int), hipMemcpyHostToDevice, stream),
-              "hipMemcpyAsync(ntiles)");
+        check(hipMemcpyAsync(d_first, h_first, sizeof(int) * nfirst, hipMemcpyHostToDevice,
+                             stream),
+              "hipMemcpyAsync(first)");
     }
 
-    BatchTables tables() const { return BatchTables{d_insts, d_ctrl, d_tiles, d_ntiles}; }
+    BatchTables tables() const { return BatchTables{d_insts, d_ctrl, d_tiles, d_first}; }
 
     template <typename R>
     void ensure_graph()
@@ -648,14 +624,18 @@ struct mhd_hip_batch {
         check(e, "hipGraphInstantiate");
     }
 
-    // Timed tuning (rows per wave -1): times one stage-1 pass (which leaves U untouched) over
-    // the current instance set for each candidate length and keeps the fastest.
+    // Timed choice (rows per wave -1): times one stage-1 pass (which leaves U untouched) over
+    // the current instance set for the capacity fill and each fixed row length, and keeps the
+    // fastest partition.
     template <typename R>
     void autotune()
     {
-        int best = 32;
+        std::vector<int> modes;
+        for (const int m : kFillMinRows) modes.push_back(-m);
+        for (const int rows : kTimedRowCandidates) modes.push_back(rows);
+        int best = modes.front();
         float best_ms = 1e30f;
-        for (const int rows : kTimedRowCandidates) {
+        for (const int rows : modes) {
             build_tables(rows);
             check(mhdhip::launch_batch_stage<R>(1, limiter, nonideal, tables(), 0, blocks, stream),
                   "tune");
@@ -760,29 +740,69 @@ struct mhd_hip_batch {
         }
     }
 
+    // Step counts and the stepping set of the replay just launched, for the progress check.
+    struct Launched {
+        std::vector<int> steps;
+        std::vector<char> stepping;
+        bool valid = false;
+    };
+
+    void note_launch(Launched& l) const
+    {
+        l.steps.assign(static_cast<size_t>(max_instances), 0);
+        l.stepping.assign(static_cast<size_t>(max_instances), 0);
+        for (int si = 0; si < max_instances; ++si) {
+            const Slot& s = slots[si];
+            l.stepping[si] = s.used && !s.finished && !s.paused;
+            l.steps[si] = h_ctrl[si].steps[0];
+        }
+        l.valid = true;
+    }
+
+    // After read_ctrl: a replay that advanced none of the instances it stepped (and finished
+    // none) would repeat forever; stop with an error instead.
+    void check_progress(const Launched& l) const
+    {
+        if (!l.valid) return;
+        std::vector<int> after(static_cast<size_t>(max_instances), 0);
+        std::vector<char> fin(static_cast<size_t>(max_instances), 0);
+        for (int si = 0; si < max_instances; ++si) {
+            after[si] = h_ctrl[si].steps[0];
+            fin[si] = slots[si].used && slots[si].finished;
+        }
+        if (!mhdhip::replay_progressed(l.steps, after, l.stepping, fin)) {
+            throw std::runtime_error(
+                "batch advance: a replay advanced none of the instances it stepped");
+        }
+    }
+
     template <typename R>
     int advance_t(int max_replays)
     {
         check(hipSetDevice(device), "hipSetDevice");
         check(hipEventRecord(ev0, stream), "hipEventRecord");
         int active = 0;
+        Launched last;
         for (int rep = 0; max_replays < 0 || rep < max_replays; ++rep) {
             read_ctrl();
+            check_progress(last);
             schedule();
             active = 0;
             for (const auto& s : slots) {
                 active += (s.used && !s.finished && !s.paused) ? 1 : 0;
             }
             if (active == 0) {
+                last.valid = false;
                 break;
             }
             if (seg_rows < 0 && tuned_rows == 0) {
                 autotune<R>();
             }
             if (dirty) {
-                build_tables(rows_for_tiles());
+                build_tables(plan_mode());
                 dirty = false;
             }
+            note_launch(last);
             if (kernel_timing) {
                 timed_replay<R>();
             } else {
@@ -796,6 +816,7 @@ struct mhd_hip_batch {
         check(hipEventElapsedTime(&ms, ev0, ev1), "hipEventElapsedTime");
         last_seconds = ms * 1e-3;
         read_ctrl();
+        check_progress(last);
         active = 0;
         for (const auto& s : slots) {
             active += (s.used && !s.finished && !s.paused) ? 1 : 0;

@@ -1,16 +1,21 @@
 // Batched stage kernel: many independent grids of different sizes advance in one persistent
 // launch per RK stage, each with its own dt, gamma, CFL number, boundary types and end time.
 //
-// The work list is a device table of tiles (instance, strip, row range); every wave of the
-// persistent grid walks the list with a stride of the total wave count, after the XCD-aware
-// remap of its workgroup, so neighbouring tiles meet in one L2. The kernel's arguments are
-// table pointers only: the host may add and remove instances between launches (by copies on
-// the same stream) and a captured graph stays valid.
+// The work list is a device table of tiles (instance, strip, row range) grouped by wave: wave
+// w of the persistent grid (its logical index after the XCD-aware remap of its workgroup)
+// marches tiles[first[w] .. first[w + 1]), a list the host plans so that every wave carries
+// about the same work and consecutive waves, which share an XCD, take neighbouring tiles. The
+// kernel's arguments are table pointers only: the host may add and remove instances between
+// launches (by copies on the same stream) and a captured graph stays valid.
 #pragma once
 
+#include "mhd_batch_plan.hpp"
 #include "mhd_kernels.hpp"
 
 namespace mhdhip {
+
+static_assert(kPlanStripCells == kStripCells && kPlanHalfCells == kHalfCells,
+              "the host planner and the kernels must agree on the strip widths");
 
 struct BatchInst {
     void* U;       // this instance's 9 planes of U
@@ -21,19 +26,11 @@ struct BatchInst {
     int pad;
 };
 
-struct BatchTile {
-    int inst;
-    int strip;
-    int r0, r1;   // global rows [r0, r1) marched by the wave (by its lower half when r0b >= 0)
-    int r0b, r1b; // half-wave tile of a narrow strip: the upper half's rows start at r0b and
-                  // are updated below r1b; r0b < 0 for a whole-wave tile
-};
-
 struct BatchTables {
     const BatchInst* insts;
     Ctrl* ctrl;  // one control block per instance slot
     const BatchTile* tiles;
-    const int* ntiles;
+    const int* first;  // per-wave offsets into tiles: one more than the waves of the launch
 };
 
 // Waves per SIMD each batched stage kernel is compiled for: what its registers allow (fp32
@@ -50,10 +47,9 @@ void k_batch_stage(BatchTables tb, int parity)
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = wave_in_block();
     const int gw = xcd_logical_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
-    const int nw = gridDim.x * kWavesPerBlock;
-    const int n = *tb.ntiles;
+    const int t_end = tb.first[gw + 1];
     __shared__ R ring[kWavesPerBlock][3][kNC][kWave];
-    for (int t = gw; t < n; t += nw) {
+    for (int t = tb.first[gw]; t < t_end; ++t) {
         const BatchTile tile = tb.tiles[t];
         const BatchInst& in = tb.insts[tile.inst];
         if (!in.enabled) {

@@ -1,0 +1,250 @@
+// Host-side planning of the batched engine, free of HIP so that it compiles and is tested on
+// the CPU as well:
+//   * the work partition: which tiles (instance, strip, row range) each wave of the
+//     persistent stage kernel marches, as per-wave lists (CSR offsets into a tile array);
+//   * the accounting of a replay of timed steps: how many of its launches stepped anything;
+//   * the progress check that turns a replay that advanced no stepping instance into an error.
+#pragma once
+
+#include <algorithm>
+#include <cstddef>
+#include <cstdint>
+#include <limits>
+#include <vector>
+
+namespace mhdhip {
+
+constexpr int kPlanStripCells = 60;  // cells per wave strip (64 lanes, 2 halo lanes each side)
+constexpr int kPlanHalfCells = 28;   // cells per half-wave strip (narrow tail strips)
+constexpr int kPlanGhosts = 2;
+
+struct BatchTile {
+    int inst;
+    int strip;
+    int r0, r1;   // global rows [r0, r1) marched by the wave (by its lower half when r0b >= 0)
+    int r0b, r1b; // half-wave tile of a narrow strip: the upper half's rows start at r0b and
+                  // are updated below r1b; r0b < 0 for a whole-wave tile
+};
+
+// One instance as the planner sees it: its slot and interior size.
+struct PlanInst {
+    int slot;
+    int nx, ny;
+};
+
+// The strips of an nx x ny grid: `wide` strips of up to 60 columns marched over all ny rows,
+// and, when the last strip is at most 28 columns wide, that strip as a half-wave strip whose
+// wave pairs a chunk of its lower ceil(ny/2) rows (lanes 0..31) with the same chunk of its
+// upper rows (lanes 32..63); it then has `half_rows` = ceil(ny/2) rows to march.
+struct StripShape {
+    int wide;       // whole-wave strips
+    bool half;      // a half-wave tail strip follows them (strip index `wide`)
+    int half_rows;  // rows the half-wave strip marches
+};
+
+inline StripShape strip_shape(int nx, int ny)
+{
+    const int full = nx / kPlanStripCells;
+    const int tail = nx - full * kPlanStripCells;
+    StripShape s{};
+    s.half = tail > 0 && tail <= kPlanHalfCells;
+    s.wide = s.half ? full : (nx + kPlanStripCells - 1) / kPlanStripCells;
+    s.half_rows = s.half ? (ny + 1) / 2 : 0;
+    return s;
+}
+
+// Tile marching rows [a0, a1) (counted from the first interior row) of strip `st`.
+inline BatchTile make_tile(int slot, int nx, int ny, int st, int a0, int a1)
+{
+    const StripShape s = strip_shape(nx, ny);
+    const int ng = kPlanGhosts;
+    if (s.half && st == s.wide) {
+        const int h = s.half_rows;
+        const int r0b = ng + h + a0;
+        return BatchTile{slot, st, ng + a0, ng + a1, r0b, std::min(r0b + (a1 - a0), ng + ny)};
+    }
+    return BatchTile{slot, st, ng + a0, ng + a1, -1, 0};
+}
+
+// Cost of a tile of L rows in half-rows: it marches L + 2 rows and loads L + 3.
+inline long long tile_cost(int rows) { return 2ll * rows + 5; }
+
+// Per-wave tile lists: the tiles of wave w are tiles[first[w] .. first[w + 1]).
+struct WavePlan {
+    std::vector<BatchTile> tiles;
+    std::vector<int> first;  // waves + 1 offsets
+    long long max_cost = 0;  // the busiest wave's cost (half-rows)
+    long long total_cost = 0;
+};
+
+// Fixed segments of `rows` rows over every strip, tile k handed to wave k mod `waves` (the
+// persistent kernel's round-robin over the whole tile list).
+inline WavePlan plan_fixed(const std::vector<PlanInst>& insts, int rows, int waves)
+{
+    rows = std::max(1, rows);
+    waves = std::max(1, waves);
+    std::vector<BatchTile> all;
+    for (const PlanInst& in : insts) {
+        const StripShape s = strip_shape(in.nx, in.ny);
+        for (int a0 = 0; a0 < in.ny; a0 += rows) {
+            for (int st = 0; st < s.wide; ++st) {
+                all.push_back(make_tile(in.slot, in.nx, in.ny, st, a0, std::min(a0 + rows, in.ny)));
+            }
+        }
+        for (int a0 = 0; a0 < s.half_rows; a0 += rows) {
+            all.push_back(
+                make_tile(in.slot, in.nx, in.ny, s.wide, a0, std::min(a0 + rows, s.half_rows)));
+        }
+    }
+    WavePlan p;
+    std::vector<int> count(static_cast<size_t>(waves), 0);
+    std::vector<long long> load(static_cast<size_t>(waves), 0);
+    for (size_t k = 0; k < all.size(); ++k) {
+        const size_t w = k % static_cast<size_t>(waves);
+        ++count[w];
+        load[w] += tile_cost(all[k].r1 - all[k].r0);
+        p.total_cost += tile_cost(all[k].r1 - all[k].r0);
+    }
+    p.first.assign(static_cast<size_t>(waves) + 1, 0);
+    for (int w = 0; w < waves; ++w) {
+        p.first[w + 1] = p.first[w] + count[w];
+    }
+    p.tiles.resize(all.size());
+    std::vector<int> pos(p.first.begin(), p.first.end() - 1);
+    for (size_t k = 0; k < all.size(); ++k) {
+        p.tiles[pos[k % static_cast<size_t>(waves)]++] = all[k];
+    }
+    p.max_cost = load.empty() ? 0 : *std::max_element(load.begin(), load.end());
+    return p;
+}
+
+// Capacity fill: the strips of all instances, in instance order, are cut into row pieces and
+// poured into the waves in order, each wave filled up to a common capacity C (half-rows,
+// piece overheads included), so every wave carries about the same work in one round of the
+// persistent grid. Consecutive waves take consecutive pieces (the same instance, neighbouring
+// strips), and consecutive logical waves share an XCD, so an instance's halo rows and strip
+// edges meet in one L2. C is the least capacity for which the pour fits in `waves` waves.
+// Pieces are at least `min_rows` rows unless a strip's remainder is shorter.
+inline WavePlan plan_fill(const std::vector<PlanInst>& insts, int waves, int min_rows = 2)
+{
+    waves = std::max(1, waves);
+    min_rows = std::max(1, min_rows);
+    struct Strip {
+        int inst;  // index into insts
+        int st;
+        int rows;
+    };
+    std::vector<Strip> strips;
+    for (size_t i = 0; i < insts.size(); ++i) {
+        const StripShape s = strip_shape(insts[i].nx, insts[i].ny);
+        for (int st = 0; st < s.wide; ++st) {
+            strips.push_back(Strip{static_cast<int>(i), st, insts[i].ny});
+        }
+        if (s.half) {
+            strips.push_back(Strip{static_cast<int>(i), s.wide, s.half_rows});
+        }
+    }
+    WavePlan p;
+    p.first.assign(static_cast<size_t>(waves) + 1, 0);
+    if (strips.empty()) {
+        return p;
+    }
+    // Pour with capacity C. Returns the waves it needs (more than `waves`: C is too small).
+    // With `out`, writes the tiles and the offsets; the waves past the last one stay empty.
+    auto pour = [&](long long C, std::vector<BatchTile>* out, std::vector<int>* first,
+                    long long* max_cost) {
+        int w = 0;
+        long long used = 0;
+        long long worst = 0;
+        for (const Strip& s : strips) {
+            int a = 0;
+            while (a < s.rows) {
+                const int left = s.rows - a;
+                const int need = std::min(left, min_rows);
+                long long fit = (C - used - 5) / 2;  // rows that fit in what this wave has left
+                if (fit < need) {
+                    if (used > 0 && w + 1 < waves + (out ? 0 : 1)) {
+                        // this wave is full: close it and pour into the next one
+                        worst = std::max(worst, used);
+                        if (first) (*first)[static_cast<size_t>(w) + 1] = static_cast<int>(out->size());
+                        ++w;
+                        used = 0;
+                        if (w >= waves) return w + 1;
+                        continue;
+                    }
+                    fit = need;  // an empty wave (or the last one) takes the piece anyway
+                }
+                int L = static_cast<int>(std::min<long long>(left, fit));
+                if (left - L > 0 && left - L < min_rows) {
+                    // leave no stub shorter than min_rows behind
+                    L = left - min_rows >= need ? left - min_rows : left;
+                }
+                if (out) {
+                    const PlanInst& in = insts[static_cast<size_t>(s.inst)];
+                    out->push_back(make_tile(in.slot, in.nx, in.ny, s.st, a, a + L));
+                }
+                used += tile_cost(L);
+                a += L;
+            }
+        }
+        worst = std::max(worst, used);
+        if (first) {
+            for (int k = w + 1; k <= waves; ++k) {
+                (*first)[static_cast<size_t>(k)] = static_cast<int>(out->size());
+            }
+        }
+        if (max_cost) *max_cost = worst;
+        return w + 1;
+    };
+    long long total = 0;
+    for (const Strip& s : strips) total += tile_cost(s.rows);
+    long long lo = std::max<long long>(tile_cost(min_rows), (total + waves - 1) / waves);
+    long long hi = total + 5;  // one wave takes everything
+    while (lo < hi) {
+        const long long mid = lo + (hi - lo) / 2;
+        if (pour(mid, nullptr, nullptr, nullptr) <= waves) {
+            hi = mid;
+        } else {
+            lo = mid + 1;
+        }
+    }
+    p.tiles.clear();
+    pour(lo, &p.tiles, &p.first, &p.max_cost);
+    for (const BatchTile& t : p.tiles) p.total_cost += tile_cost(t.r1 - t.r0);
+    return p;
+}
+
+// ---- replay accounting ---------------------------------------------------------------------
+// Launches of a replay of `steps` steps that stepped some instance: the largest step advance
+// of the instances stepping in it (the later launches found every instance finished and
+// returned at once). before/after: step counts at the start and end of the replay;
+// stepping: which entries took part.
+inline int replay_active_launches(const std::vector<int>& before, const std::vector<int>& after,
+                                  const std::vector<char>& stepping, int steps)
+{
+    int most = 0;
+    for (size_t k = 0; k < before.size() && k < after.size() && k < stepping.size(); ++k) {
+        if (stepping[k]) {
+            most = std::max(most, after[k] - before[k]);
+        }
+    }
+    return std::min(std::max(most, 0), steps);
+}
+
+// True when a replay advanced (or finished) some instance that was stepping in it. A replay
+// that leaves every stepping instance where it was can only repeat itself: the host must stop
+// with an error instead of replaying forever.
+inline bool replay_progressed(const std::vector<int>& before, const std::vector<int>& after,
+                              const std::vector<char>& stepping,
+                              const std::vector<char>& finished_after)
+{
+    bool any = false;
+    for (size_t k = 0; k < stepping.size(); ++k) {
+        if (!stepping[k]) continue;
+        any = true;
+        if (after[k] > before[k] || finished_after[k]) return true;
+    }
+    return !any;  // nothing was stepping: nothing to advance
+}
+
+}  // namespace mhdhip

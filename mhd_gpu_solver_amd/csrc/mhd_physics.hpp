@@ -243,17 +243,19 @@ __device__ __forceinline__ void hll_face(R (&WL)[kNC], R (&WR)[kNC], const R (&W
     if constexpr (sizeof(R) == 4) {
         // fp32: with sL clamped to <= 0 and sR to >= the smallest normal float, the HLL formula
         // itself returns FL (sL >= 0, also when both speeds vanish) or FR (sR <= 0), to
-        // rounding, so no per-component selects are needed. sR - sL >= 2 c_h > 0 for
-        // admissible states, so the reference's average for a span below 1e-14 with
-        // sL < 0 < sR is not reached; such a span would blend FL and FR by the speeds.
-        (void)use_left;
-        (void)use_right;
-        const R sLc = vmin(sL, zero);
-        const R sRc = vmax(sR, R(1.17549435e-38f));
+        // rounding, so no per-component selects are needed. A span below 1e-14 with
+        // sL < 0 < sR (every signal speed of the grid below 5e-15) takes the reference's
+        // average (FL + FR) / 2 through the same formula, with weights 1/2, 1/2 and no
+        // dissipation term.
+        const bool degenerate = !use_left && !use_right && (sR - sL) < Lit<R>::kDegenerate;
+        // the degenerate lanes weigh FL and FR by 1 / 2 each (speeds -1 and 1) and drop the
+        // dissipation term
+        const R sLc = degenerate ? R(-1) : vmin(sL, zero);
+        const R sRc = degenerate ? R(1) : vmax(sR, R(1.17549435e-38f));
         const R inv_denom = drcp(sRc - sLc);
         const R a = sRc * inv_denom;
         const R b = -sLc * inv_denom;
-        const R k = sLc * sRc * inv_denom;
+        const R k = degenerate ? zero : sLc * sRc * inv_denom;
 #pragma unroll
         for (int c = 0; c < kNC; ++c) {
             F[c] = a * FL[c] + (b * FR[c] + k * (UR[c] - UL[c]));

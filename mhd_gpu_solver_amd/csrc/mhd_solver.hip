@@ -661,6 +661,7 @@ struct mhd_hip_solver {
                                             : -1;
                 int slot = 0;
                 long long issued = 0;
+                int seen_steps = -1;  // step count after the previous replay read back
                 launch_one(slot);
                 ++issued;
                 while (true) {
@@ -677,6 +678,11 @@ struct mhd_hip_solver {
                     if (finished && fixed < 0) {
                         break;
                     }
+                    // an unfinished solve whose replay took no step would replay forever
+                    if (!finished && c.steps[0] <= seen_steps) {
+                        throw std::runtime_error("solve: a replay of steps made no progress");
+                    }
+                    seen_steps = c.steps[0];
                     slot ^= 1;
                 }
                 check(hipStreamSynchronize(stream), "hipStreamSynchronize");

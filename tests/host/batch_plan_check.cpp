@@ -1,0 +1,145 @@
+// CPU checks of the batched engine's host planning (mhd_gpu_solver_amd/csrc/mhd_batch_plan.hpp):
+// every interior cell of every instance is marched exactly once, each instance has exactly one
+// lead tile (its step bookkeeping), the per-wave offsets are consistent, the capacity fill
+// balances the waves, and the replay accounting / progress check behave on fake step counts.
+// Built and run by tests/test_batch_plan.py; prints "ok" and exits 0 when every check holds.
+#include "mhd_batch_plan.hpp"
+
+#include <cstdio>
+#include <cstdlib>
+#include <random>
+#include <vector>
+
+using namespace mhdhip;
+
+static int failures = 0;
+#define CHECK(cond, ...)                                  \
+    do {                                                  \
+        if (!(cond)) {                                    \
+            std::printf("FAIL %s:%d: ", __FILE__, __LINE__); \
+            std::printf(__VA_ARGS__);                     \
+            std::printf("\n");                            \
+            ++failures;                                   \
+        }                                                 \
+    } while (0)
+
+// each (row, strip) of every instance covered once; one lead tile per instance
+static void check_cover(const std::vector<PlanInst>& insts, const WavePlan& p, int waves,
+                        const char* what)
+{
+    CHECK(static_cast<int>(p.first.size()) == waves + 1, "%s: offsets size", what);
+    CHECK(p.first.front() == 0 && p.first.back() == static_cast<int>(p.tiles.size()),
+          "%s: offsets ends", what);
+    for (int w = 0; w < waves; ++w) CHECK(p.first[w] <= p.first[w + 1], "%s: offsets order", what);
+    for (const PlanInst& in : insts) {
+        const StripShape s = strip_shape(in.nx, in.ny);
+        const int nstrips = s.wide + (s.half ? 1 : 0);
+        std::vector<int> hits(static_cast<size_t>(nstrips) * in.ny, 0);
+        int leads = 0;
+        for (const BatchTile& t : p.tiles) {
+            if (t.inst != in.slot) continue;
+            CHECK(t.strip >= 0 && t.strip < nstrips, "%s: strip index", what);
+            CHECK(t.r0 >= kPlanGhosts && t.r0 < t.r1 && t.r1 <= kPlanGhosts + in.ny,
+                  "%s: rows %d..%d of %d", what, t.r0, t.r1, in.ny);
+            if (t.strip == 0 && t.r0 == kPlanGhosts) ++leads;
+            const bool half = s.half && t.strip == s.wide;
+            CHECK(half == (t.r0b >= 0), "%s: tile kind", what);
+            for (int r = t.r0; r < t.r1; ++r) {
+                if (half && r - kPlanGhosts >= s.half_rows) continue;
+                ++hits[static_cast<size_t>(t.strip) * in.ny + (r - kPlanGhosts)];
+            }
+            if (half) {
+                CHECK(t.r0b - t.r0 == s.half_rows, "%s: half offset", what);
+                for (int r = t.r0b; r < t.r1b; ++r) {
+                    ++hits[static_cast<size_t>(t.strip) * in.ny + (r - kPlanGhosts)];
+                }
+            }
+        }
+        CHECK(leads == 1, "%s: %d lead tiles for slot %d", what, leads, in.slot);
+        for (size_t k = 0; k < hits.size(); ++k) {
+            if (hits[k] != 1) {
+                CHECK(false, "%s: slot %d (%dx%d) strip %zu row %zu marched %d times", what,
+                      in.slot, in.nx, in.ny, k / in.ny, k % in.ny, hits[k]);
+                break;
+            }
+        }
+    }
+}
+
+int main()
+{
+    std::mt19937 rng(12345);
+    // edge shapes: one-cell grids, narrow tails of 1..28 columns, 1..3 rows, wide grids
+    std::vector<std::vector<PlanInst>> sets;
+    sets.push_back({{0, 1, 1}});
+    sets.push_back({{3, 61, 1}, {5, 88, 2}, {6, 28, 3}, {9, 121, 9}, {1, 5, 40}});
+    sets.push_back({{0, 4096, 4096}});
+    sets.push_back({{0, 64, 64}, {1, 96, 96}, {2, 128, 128}, {3, 192, 192}, {4, 256, 256}});
+    {
+        // the mixed-size ensemble's shapes
+        std::vector<PlanInst> ens;
+        const int sizes[5] = {64, 96, 128, 192, 256};
+        for (int k = 0; k < 512; ++k) ens.push_back({k, sizes[rng() % 5], 0});
+        for (auto& e : ens) e.ny = e.nx;
+        sets.push_back(ens);
+    }
+    {
+        std::vector<PlanInst> rnd;
+        for (int k = 0; k < 40; ++k) rnd.push_back({k, 1 + static_cast<int>(rng() % 300),
+                                                    1 + static_cast<int>(rng() % 300)});
+        sets.push_back(rnd);
+    }
+    const int wave_counts[] = {1, 3, 64, 2048, 4096};
+    for (const auto& set : sets) {
+        for (const int waves : wave_counts) {
+            for (const int rows : {1, 5, 32, 200}) {
+                check_cover(set, plan_fixed(set, rows, waves), waves, "fixed");
+            }
+            for (const int min_rows : {1, 2, 4}) {
+                const WavePlan p = plan_fill(set, waves, min_rows);
+                check_cover(set, p, waves, "fill");
+                // balance: the busiest wave carries at most the mean plus one piece's overhead
+                // and one minimum piece, unless one strip alone is longer (it is cut anyway)
+                const long long mean = (p.total_cost + waves - 1) / waves;
+                CHECK(p.max_cost <= mean + tile_cost(min_rows) + 5 + 2 * min_rows ||
+                          p.max_cost <= tile_cost(min_rows) + 5 + 2 * min_rows,
+                      "fill balance: max %lld mean %lld (waves %d, min_rows %d)", p.max_cost,
+                      mean, waves, min_rows);
+            }
+        }
+    }
+    // the ensemble: the fill beats round-robin fixed segments on the busiest wave
+    {
+        const auto& ens = sets[4];
+        const WavePlan f = plan_fill(ens, 4096, 2);
+        long long best_fixed = -1;
+        for (const int rows : {8, 16, 24, 32, 48, 64}) {
+            const long long c = plan_fixed(ens, rows, 4096).max_cost;
+            best_fixed = best_fixed < 0 ? c : std::min(best_fixed, c);
+        }
+        std::printf("ensemble 4096 waves: fill max %lld (mean %lld), best fixed max %lld\n",
+                    f.max_cost, f.total_cost / 4096, best_fixed);
+        CHECK(f.max_cost < best_fixed, "fill %lld not below fixed %lld", f.max_cost, best_fixed);
+    }
+    // replay accounting
+    {
+        const std::vector<int> before = {0, 16, 5, 9};
+        const std::vector<int> after = {8, 20, 5, 9};
+        const std::vector<char> stepping = {1, 1, 0, 1};
+        CHECK(replay_active_launches(before, after, stepping, 8) == 8, "active launches 8");
+        CHECK(replay_active_launches({0, 16}, {3, 20}, {1, 1}, 8) == 4, "active launches 4");
+        CHECK(replay_active_launches({4}, {4}, {1}, 8) == 0, "active launches 0");
+        CHECK(replay_active_launches({0}, {30}, {1}, 8) == 8, "clamped to the replay");
+        CHECK(replay_progressed(before, after, stepping, {0, 0, 0, 0}), "progress");
+        CHECK(!replay_progressed({3, 7}, {3, 7}, {1, 1}, {0, 0}), "no progress");
+        CHECK(replay_progressed({3, 7}, {3, 7}, {1, 1}, {0, 1}), "finished counts");
+        CHECK(replay_progressed({3, 7}, {3, 7}, {0, 0}, {0, 0}), "nothing stepping");
+        CHECK(!replay_progressed({3, 7}, {3, 9}, {1, 0}, {0, 0}), "only a paused one moved");
+    }
+    if (failures) {
+        std::printf("%d failures\n", failures);
+        return 1;
+    }
+    std::printf("ok\n");
+    return 0;
+}

@@ -308,3 +308,46 @@ def test_periodic_source_small_n(oracle):
     assert [ps(g, 1, 2, 0) for g in range(2)] == [2, 2]
     assert [ps(g, 5, 2, 1) for g in range(2)] == [5, 6]
     assert [ps(g, 5, 2, 0) for g in range(2)] == [2, 3]
+
+
+# ---- rare branches of the face kernel on whole solves ---------------------------------------
+
+def test_fallback_and_degenerate_span_solves_bitexact_vs_reference(oracle, reference):
+    """The states that force the positivity fallback (a block of p < 0) and the degenerate
+    wave-speed span (a cold flow with c_h < 5e-15) solve for a few steps to the reference's
+    bits, with finite states; c_h of the cold flow stays below the 1e-14 span bound."""
+    from rare_states import cold_slow_flow, negative_pressure_patch
+    g = make_grid(48, 40)
+    for name, U0, lim in (("fallback", negative_pressure_patch(oracle, g), MC),
+                          ("degenerate", cold_slow_flow(oracle, g), MINMOD)):
+        p = make_params(t_end=1e300, limiter=lim, max_steps=4)
+        a, ia = oracle.solve(g, U0, p, allow_max_steps=True, dt_cap=8)
+        b, ib = reference.solve(g, U0, p, allow_max_steps=True, dt_cap=8)
+        assert np.isfinite(a).all(), name
+        assert ia["steps"] == ib["steps"] == 4 and ia["c_h"] == ib["c_h"], name
+        np.testing.assert_array_equal(ia["dt"], ib["dt"])
+        assert np.array_equal(a, b), name
+        if name == "degenerate":
+            assert 0.0 < ia["c_h"] < 5e-15
+            assert not np.array_equal(a, U0)   # the flow moved
+
+
+def test_degenerate_span_average_matches_closed_form(oracle):
+    """A face whose speeds span less than 1e-14 (sL < 0 < sR) returns (FL + FR) / 2 exactly,
+    where the HLL formula would add the dissipation sL sR (UR - UL) / (sR - sL)."""
+    W = np.zeros((4, 9))
+    W[:, 0] = [1.0, 1.1, 0.9, 1.0]
+    W[:, 4] = [1e-32, 2e-32, 1.5e-32, 1e-32]
+    W[:, 1] = [1e-18, -1e-18, 2e-18, 0.0]
+    F = oracle.face_flux(MINMOD, *W, 5.0 / 3.0, 1e-16)
+    # minmod reconstruction of the face between cells 1 and 2
+    def lim(a, b):
+        return 0.0 if a * b <= 0 else (a if abs(a) < abs(b) else b)
+    WL = W[1] + 0.5 * np.array([lim(W[1, k] - W[0, k], W[2, k] - W[1, k]) for k in range(9)])
+    WR = W[2] - 0.5 * np.array([lim(W[2, k] - W[1, k], W[3, k] - W[2, k]) for k in range(9)])
+
+    def flux(w):
+        rho, vn, p = w[0], w[1], w[4]
+        e = p / (5.0 / 3.0 - 1.0) + 0.5 * rho * vn * vn
+        return np.array([rho * vn, rho * vn * vn + p, 0, 0, (e + p) * vn, 0, 0, 0, 0])
+    np.testing.assert_allclose(F, 0.5 * (flux(WL) + flux(WR)), rtol=1e-14, atol=1e-60)
