@@ -297,8 +297,8 @@ int mhd_hip_batch_rel_l1(mhd_hip_batch* a, int id_a, mhd_hip_batch* ref, int id_
  * stepping set changes: every strip is cut into row pieces poured into the waves in order up
  * to a common capacity, so all waves carry about the same work; -1: the fill and fixed row
  * lengths are each timed once (one stage-1 pass) on the first advance's set and the fastest
- * is kept; > 0: fixed segments of that many rows handed out round-robin. Results do not depend
- * on this value. */
+ * is kept; > 0: fixed segments of that many rows handed out round-robin; < -1: the capacity
+ * fill with pieces of at least -rows rows. Results do not depend on this value. */
 int mhd_hip_batch_set_seg_rows(mhd_hip_batch* b, int rows);
 
 int mhd_hip_batch_info(const mhd_hip_batch* b, mhd_hip_batch_stats* stats);

@@ -161,7 +161,8 @@ struct mhd_hip_batch {
     int sched_policy = MHD_HIP_SCHED_ALL;
     long long sched_cap = 0;  // interior cells stepping per replay under SRPT
     // Work partition: > 0 fixed segments of that many rows, handed out round-robin; 0 the
-    // capacity fill (plan_fill) at every table build; -1 timed once over the candidates.
+    // capacity fill (plan_fill) at every table build; -1 timed once over the candidates;
+    // < -1 the capacity fill with pieces of at least -seg_rows rows.
     int seg_rows = 0;
     int tuned_rows = 0;  // the timed choice: > 0 fixed rows, < 0 fill with -tuned_rows min rows
     int last_rows = 0;  // fixed rows of the current tile list (0: capacity fill)
@@ -550,8 +551,8 @@ struct mhd_hip_batch {
 
     int plan_mode() const
     {
-        if (seg_rows > 0) return seg_rows;
-        if (seg_rows < 0 && tuned_rows != 0) return tuned_rows;
+        if (seg_rows > 0 || seg_rows < -1) return seg_rows;
+        if (seg_rows == -1 && tuned_rows != 0) return tuned_rows;
         return 0;
     }
 
@@ -795,7 +796,7 @@ struct mhd_hip_batch {
                 last.valid = false;
                 break;
             }
-            if (seg_rows < 0 && tuned_rows == 0) {
+            if (seg_rows == -1 && tuned_rows == 0) {
                 autotune<R>();
             }
             if (dirty) {
@@ -1056,7 +1057,6 @@ int mhd_hip_batch_set_seg_rows(mhd_hip_batch* b, int rows)
 {
     return guarded([&] {
         if (!b) throw InvalidArgument("null batch");
-        if (rows < -1) throw InvalidArgument("rows per wave segment must be >= -1");
         b->seg_rows = rows;
         b->tuned_rows = 0;
         b->dirty = true;
