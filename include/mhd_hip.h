@@ -222,7 +222,7 @@ typedef struct mhd_hip_batch_stats {
     int instances; /* admitted (running or finished, not yet removed) */
     int active;    /* still stepping (unfinished, not paused) */
     int tiles;     /* wave work items in the current tile list */
-    int seg_rows;  /* fixed rows per tile of the current tile list; 0: capacity fill */
+    int seg_rows;  /* rows per tile of the current tile list (0: capacity fill) */
     int blocks;    /* workgroups of the persistent stage kernel */
     size_t bytes_used, bytes_budget;
     double last_seconds; /* device time of the last mhd_hip_batch_advance */
@@ -293,12 +293,15 @@ int mhd_hip_batch_rel_l1(mhd_hip_batch* a, int id_a, mhd_hip_batch* ref, int id_
                          double l1[9]);
 
 /* Work partition of the batch kernel. Each wave of the persistent grid marches its own list
- * of tiles (instance, strip, row range). 0 (default): capacity fill, re-planned whenever the
- * stepping set changes: every strip is cut into row pieces poured into the waves in order up
- * to a common capacity, so all waves carry about the same work; -1: the fill and fixed row
- * lengths are each timed once (one stage-1 pass) on the first advance's set and the fastest
- * is kept; > 0: fixed segments of that many rows handed out round-robin; < -1: the capacity
- * fill with pieces of at least -rows rows. Results do not depend on this value. */
+ * of tiles (instance, strip, row range). 0 (default): fixed segments whose length a load
+ * model picks whenever the stepping set changes (it replays the round-robin hand-out of the
+ * tiles to the waves for candidate lengths and keeps the one whose busiest wave has the least
+ * work; suits a set that changes, as in serving); -1: the candidate lengths and the capacity
+ * fill are each timed once (one stage-1 pass) on the first advance's set and the fastest is
+ * kept (suits a set that stays the same); > 0: fixed segments of that many rows handed out
+ * round-robin; < -1: the capacity fill, every strip cut into row pieces of at least -rows rows
+ * poured into the waves in order up to a common capacity. Results do not depend on this
+ * value. */
 int mhd_hip_batch_set_seg_rows(mhd_hip_batch* b, int rows);
 
 int mhd_hip_batch_info(const mhd_hip_batch* b, mhd_hip_batch_stats* stats);

@@ -160,9 +160,9 @@ struct mhd_hip_batch {
     bool dirty = true;
     int sched_policy = MHD_HIP_SCHED_ALL;
     long long sched_cap = 0;  // interior cells stepping per replay under SRPT
-    // Work partition: > 0 fixed segments of that many rows, handed out round-robin; 0 the
-    // capacity fill (plan_fill) at every table build; -1 timed once over the candidates;
-    // < -1 the capacity fill with pieces of at least -seg_rows rows.
+    // Work partition: > 0 fixed segments of that many rows, handed out round-robin; 0 the load
+    // model's length at every table build; -1 timed once over the candidates; < -1 the
+    // capacity fill with pieces of at least -seg_rows rows.
     int seg_rows = 0;
     int tuned_rows = 0;  // the timed choice: > 0 fixed rows, < 0 fill with -tuned_rows min rows
     int last_rows = 0;  // fixed rows of the current tile list (0: capacity fill)
@@ -524,17 +524,15 @@ struct mhd_hip_batch {
         dirty = true;
     }
 
-    // Fixed row lengths the timed choice (rows per wave -1) tries beside the capacity fill
-    // with minimum pieces of kFillMinRows rows (plan_fill).
+    // Row lengths the load model weighs, and those the timed choice (rows per wave -1) tries
+    // beside the capacity fill with minimum pieces of kFillMinRows rows.
+    static constexpr int kModelRowCandidates[] = {8, 12, 16, 20, 24, 28, 32, 40, 48, 64, 96, 128};
     static constexpr int kTimedRowCandidates[] = {8, 16, 24, 32, 48, 64};
-    static constexpr int kFillMinRows[] = {2, 8};
+    static constexpr int kFillMinRows[] = {32};
 
     int waves() const { return blocks * mhdhip::kWavesPerBlock; }
 
-    // Work partition of the stepping instances: `mode` > 0 fixed segments of that many rows
-    // handed out round-robin; mode <= 0 the capacity fill with minimum pieces of -mode rows
-    // (0: two rows).
-    mhdhip::WavePlan plan(int mode) const
+    std::vector<mhdhip::PlanInst> stepping_set() const
     {
         std::vector<mhdhip::PlanInst> insts;
         for (int si = 0; si < max_instances; ++si) {
@@ -543,10 +541,18 @@ struct mhd_hip_batch {
                 insts.push_back(mhdhip::PlanInst{si, s.desc.nx, s.desc.ny});
             }
         }
-        if (mode > 0) {
-            return mhdhip::plan_fixed(insts, mode, waves());
-        }
-        return mhdhip::plan_fill(insts, waves(), mode < 0 ? -mode : kFillMinRows[0]);
+        return insts;
+    }
+
+    // The partition a mode stands for on the current stepping set: > 0 fixed segments of that
+    // many rows handed out round-robin (0: the load model's length for this set); < 0 the
+    // capacity fill with pieces of at least -mode rows.
+    int resolve(int mode, const std::vector<mhdhip::PlanInst>& insts) const
+    {
+        if (mode != 0) return mode;
+        return mhdhip::model_rows(insts, waves(),
+                                  std::vector<int>(std::begin(kModelRowCandidates),
+                                                   std::end(kModelRowCandidates)));
     }
 
     int plan_mode() const
@@ -558,8 +564,11 @@ struct mhd_hip_batch {
 
     void build_tables(int mode)
     {
-        const mhdhip::WavePlan p = plan(mode);
-        last_rows = mode > 0 ? mode : 0;
+        const std::vector<mhdhip::PlanInst> insts = stepping_set();
+        const int r = resolve(mode, insts);
+        const mhdhip::WavePlan p = r < 0 ? mhdhip::plan_fill(insts, waves(), -r)
+                                         : mhdhip::plan_fixed(insts, r, waves());
+        last_rows = r > 0 ? r : 0;
         const size_t n = p.tiles.size();
         ensure_tiles(std::max<size_t>(n, 1));
         check(hipStreamSynchronize(stream), "hipStreamSynchronize");  // staging reuse

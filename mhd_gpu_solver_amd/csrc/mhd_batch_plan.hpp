@@ -118,6 +118,43 @@ inline WavePlan plan_fixed(const std::vector<PlanInst>& insts, int rows, int wav
     return p;
 }
 
+// Busiest wave's cost (half-rows) of plan_fixed(insts, rows, waves), without building it.
+inline long long fixed_max_cost(const std::vector<PlanInst>& insts, int rows, int waves)
+{
+    rows = std::max(1, rows);
+    waves = std::max(1, waves);
+    std::vector<long long> load(static_cast<size_t>(waves), 0);
+    long long k = 0;
+    auto add = [&](int len) { load[static_cast<size_t>(k++ % waves)] += tile_cost(len); };
+    for (const PlanInst& in : insts) {
+        const StripShape s = strip_shape(in.nx, in.ny);
+        for (int a0 = 0; a0 < in.ny; a0 += rows) {
+            for (int st = 0; st < s.wide; ++st) add(std::min(rows, in.ny - a0));
+        }
+        for (int a0 = 0; a0 < s.half_rows; a0 += rows) add(std::min(rows, s.half_rows - a0));
+    }
+    return *std::max_element(load.begin(), load.end());
+}
+
+// Load model: the fixed row length (of `candidates`) whose round-robin hand-out leaves the
+// busiest wave the least work, the longer length on ties (fewer halo re-reads). Round-robin
+// hands neighbouring strips of one row band to consecutive waves, which share a workgroup and
+// an XCD, so the strips' shared halo lines are fetched once for all of them.
+inline int model_rows(const std::vector<PlanInst>& insts, int waves,
+                      const std::vector<int>& candidates)
+{
+    int best = candidates.empty() ? 32 : candidates.front();
+    long long best_cost = std::numeric_limits<long long>::max();
+    for (const int rows : candidates) {
+        const long long c = fixed_max_cost(insts, rows, waves);
+        if (c < best_cost || (c == best_cost && rows > best)) {
+            best_cost = c;
+            best = rows;
+        }
+    }
+    return best;
+}
+
 // Capacity fill: the strips of all instances, in instance order, are cut into row pieces and
 // poured into the waves in order, each wave filled up to a common capacity C (half-rows,
 // piece overheads included), so every wave carries about the same work in one round of the

@@ -121,6 +121,20 @@ int main()
                     f.max_cost, f.total_cost / 4096, best_fixed);
         CHECK(f.max_cost < best_fixed, "fill %lld not below fixed %lld", f.max_cost, best_fixed);
     }
+    // the load model: its cost is plan_fixed's, and it picks the cheapest candidate
+    for (const auto& set : sets) {
+        for (const int waves : {64, 4096}) {
+            const std::vector<int> cands = {8, 16, 32, 64};
+            const int pick = model_rows(set, waves, cands);
+            long long best = -1;
+            for (const int r : cands) {
+                const long long c = plan_fixed(set, r, waves).max_cost;
+                CHECK(c == fixed_max_cost(set, r, waves), "fixed_max_cost %d", r);
+                best = best < 0 ? c : std::min(best, c);
+            }
+            CHECK(fixed_max_cost(set, pick, waves) == best, "model pick %d", pick);
+        }
+    }
     // replay accounting
     {
         const std::vector<int> before = {0, 16, 5, 9};
