@@ -20,18 +20,11 @@ from .solver import (  # noqa: F401
     hip_available,
     init_ic,
     metal_available,
-    num_threads,
-    set_num_threads,
-    set_simd_enabled,
-    simd_backend,
-    simd_enabled,
-    simd_width,
     wall_plan,
 )
 
 __all__ = [
     "BatchEngine", "CapacityFull", "instance_bytes", "FIELDS", "BoundaryConditionType", "DeviceSolver", "MHDSolver", "NonIdealConfig",
     "SlopeLimiter", "SolveParams", "SolveResult", "available_backends", "cuda_available",
-    "hip_available", "init_ic", "metal_available", "wall_plan", "state_io", "num_threads",
-    "set_num_threads", "set_simd_enabled", "simd_backend", "simd_enabled", "simd_width",
+    "hip_available", "init_ic", "metal_available", "wall_plan", "state_io",
 ]

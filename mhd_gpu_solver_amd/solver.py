@@ -12,8 +12,6 @@ from __future__ import annotations
 import ctypes as C
 import dataclasses
 import enum
-import os
-import re
 
 import numpy as np
 
@@ -130,48 +128,6 @@ class SolveResult:
 
 def hip_available() -> bool:
     return bool(_lib.lib().mhd_hip_available())
-
-
-# ---- execution settings (python/mhd_pybind.cpp:169-180, include/mhd/parallel.hpp:201-256) ----
-# They steer the reference's CPU thread pool and SIMD kernels. Every step here runs on the GPU,
-# so the settings are kept, with the reference's semantics, only so that scripts which set them
-# keep working; they do not change what the hip backend computes.
-_execution = {"threads": 0, "simd": True}
-
-
-def _default_thread_count() -> int:
-    """$MHD_NUM_THREADS when it parses to a positive integer (atoi), else all cores."""
-    env = os.environ.get("MHD_NUM_THREADS", "")
-    m = re.match(r"\s*([+-]?\d+)", env)
-    n = int(m.group(1)) if m else 0
-    return n if n > 0 else (os.cpu_count() or 1)
-
-
-def set_num_threads(n: int) -> None:
-    """0 restores the default (MHD_NUM_THREADS or all cores); negative counts clamp to 0."""
-    _execution["threads"] = max(0, int(n))
-
-
-def num_threads() -> int:
-    return _execution["threads"] if _execution["threads"] > 0 else _default_thread_count()
-
-
-def set_simd_enabled(enabled: bool) -> None:
-    _execution["simd"] = bool(enabled)
-
-
-def simd_enabled() -> bool:
-    return _execution["simd"]
-
-
-def simd_backend() -> str:
-    """Instruction set of the CPU kernels: this package has none, hence "scalar"."""
-    return "scalar"
-
-
-def simd_width() -> int:
-    """Doubles per CPU SIMD vector (1: no CPU SIMD kernels here)."""
-    return 1
 
 
 def cuda_available() -> bool:

@@ -134,21 +134,9 @@ def test_server_rejects_unknown_policy():
         Server(policy="lifo")
 
 
-def test_execution_settings_keep_reference_semantics(monkeypatch):
-    """set_num_threads / num_threads / set_simd_enabled / simd_* (python/mhd_pybind.cpp:169-180,
-    include/mhd/parallel.hpp:201-256): accepted with the reference's meaning; 0 = default."""
-    import os
-    monkeypatch.setenv("MHD_NUM_THREADS", "5")
-    m.set_num_threads(0)
-    assert m.num_threads() == 5
-    m.set_num_threads(3)
-    assert m.num_threads() == 3
-    m.set_num_threads(-2)
-    assert m.num_threads() == 5
-    monkeypatch.setenv("MHD_NUM_THREADS", "junk")
-    assert m.num_threads() == (os.cpu_count() or 1)
-    m.set_simd_enabled(False)
-    assert m.simd_enabled() is False
-    m.set_simd_enabled(True)
-    assert m.simd_enabled() is True
-    assert m.simd_backend() == "scalar" and m.simd_width() == 1
+def test_no_cpu_execution_settings():
+    """The reference module's CPU thread-pool and SIMD settings (python/mhd_pybind.cpp:169-180)
+    steer a CPU solver this package does not have, so they are not offered."""
+    for name in ("set_num_threads", "num_threads", "set_simd_enabled", "simd_enabled",
+                 "simd_backend", "simd_width"):
+        assert not hasattr(m, name), name
