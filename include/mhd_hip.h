@@ -139,6 +139,17 @@ int mhd_hip_fill_boundaries(mhd_hip_solver* s, double gamma, int what);
  * mass/energy drift; the reference sums plainly on the host (diagnostics.hpp:42-57). */
 int mhd_hip_totals(mhd_hip_solver* s, double sums[9]);
 
+/* mhd_hip_totals without any allocation, on any call. No counterpart in the reference: for
+ * callers that cannot allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc). The same kernels, hence the same bits, with the partial sums and the totals in
+ * a static device array of the library (73,800 bytes per device, held until the process
+ * ends) instead of buffers the solver allocates on its first mhd_hip_totals call. The array
+ * holds the partials of a fixed number of workgroups whatever the grid, so no grid is too
+ * large for it; devices beyond index 63 are rejected (EINVAL). Every call on a device shares
+ * that device's array (each process holds its own copy), so calls that overlapped on one
+ * device would race: the library serialises them within the process. */
+int mhd_hip_totals_noalloc(mhd_hip_solver* s, double sums[9]);
+
 /* max |dBx/dx + dBy/dy| over the interior by central differences of the device state, ghosts
  * included, in fp64 with the reference's operation order and max rule; after a solve (which
  * ends with the ghost and wall fill) it equals the reference's MHDSolver::max_div_b

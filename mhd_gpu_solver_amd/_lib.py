@@ -74,6 +74,7 @@ SIGNATURES = [
     ("mhd_hip_fill_boundaries", C.c_int, [C.c_void_p, C.c_double, C.c_int]),
     ("mhd_hip_last_solve_seconds", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     ("mhd_hip_totals", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    ("mhd_hip_totals_noalloc", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     ("mhd_hip_set_seg_rows", C.c_int, [C.c_void_p, C.c_int]),
     ("mhd_hip_get_seg_rows", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     ("mhd_hip_set_kernel_timing", C.c_int, [C.c_void_p, C.c_int]),

@@ -68,6 +68,44 @@ bool device_is_gfx950(int dev)
 
 }  // namespace
 
+namespace {
+
+// Scratch of mhd_hip_totals_noalloc: the per-workgroup partial sums of k_totals_partial
+// (kTotalsBlocks x 9 double-double values, written in full by every call) and the nine
+// totals k_totals_final writes, 73,800 bytes per device for the life of the process. Host
+// code never passes `scratch` itself to a kernel or a copy: it is a host-side stand-in;
+// noalloc_scratch(device) gives that device's address of the array.
+constexpr size_t kNoallocPartials = static_cast<size_t>(mhdhip::kTotalsBlocks) * mhdhip::kNC;
+constexpr size_t kNoallocBytes = kNoallocPartials * sizeof(mhdhip::DD) + 9 * sizeof(double);
+static __device__ __align__(256) unsigned char scratch[kNoallocBytes];
+
+constexpr int kNoallocMaxDevices = 64;
+std::mutex g_noalloc_mutex;
+void* g_noalloc_ptr[kNoallocMaxDevices] = {};
+
+// The device address of `scratch` on `device`, looked up once per device with that device
+// current (each device holds its own copy of the array). Caller holds g_noalloc_mutex.
+void* noalloc_scratch(int device)
+{
+    if (device < 0 || device >= kNoallocMaxDevices) {
+        throw InvalidArgument("totals without allocation: device index " + std::to_string(device) +
+                              " beyond the " + std::to_string(kNoallocMaxDevices) +
+                              " devices the scratch table holds");
+    }
+    if (g_noalloc_ptr[device] == nullptr) {
+        check(hipSetDevice(device), "hipSetDevice");
+        void* ptr = nullptr;
+        check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (ptr == nullptr) {
+            throw std::runtime_error("hipGetSymbolAddress(scratch) returned a null address");
+        }
+        g_noalloc_ptr[device] = ptr;
+    }
+    return g_noalloc_ptr[device];
+}
+
+}  // namespace
+
 using mhdhip::Ctrl;
 using mhdhip::Gate;
 using mhdhip::Geom;
@@ -768,6 +806,30 @@ struct mhd_hip_solver {
         check(hipStreamSynchronize(stream), "hipStreamSynchronize");
     }
 
+    // totals() with its partial sums and output in the process-wide `scratch` array of this
+    // device instead of buffers the solver allocates: no allocation on any call. Calls from
+    // this process are serialised (one at a time per process, all devices).
+    void totals_noalloc(double out[9])
+    {
+        std::lock_guard<std::mutex> lk(g_noalloc_mutex);
+        char* base = static_cast<char*>(noalloc_scratch(device));
+        check(hipSetDevice(device), "hipSetDevice");
+        auto* partial = reinterpret_cast<mhdhip::DD*>(base);
+        auto* dev_out = reinterpret_cast<double*>(base + kNoallocPartials * sizeof(mhdhip::DD));
+        if (precision == MHD_HIP_FP64) {
+            check(mhdhip::launch_totals<double>(static_cast<double*>(U), geom, partial, dev_out,
+                                                stream),
+                  "totals");
+        } else {
+            check(mhdhip::launch_totals<float>(static_cast<float*>(U), geom, partial, dev_out,
+                                               stream),
+                  "totals");
+        }
+        check(hipMemcpyAsync(out, dev_out, 9 * sizeof(double), hipMemcpyDeviceToHost, stream),
+              "hipMemcpyAsync(totals)");
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    }
+
     void totals(double out[9])
     {
         check(hipSetDevice(device), "hipSetDevice");
@@ -1040,6 +1102,14 @@ int mhd_hip_totals(mhd_hip_solver* s, double sums[9])
     return guarded([&] {
         if (!s || !sums) throw InvalidArgument("null argument");
         s->totals(sums);
+    });
+}
+
+int mhd_hip_totals_noalloc(mhd_hip_solver* s, double sums[9])
+{
+    return guarded([&] {
+        if (!s || !sums) throw InvalidArgument("null argument");
+        s->totals_noalloc(sums);
     });
 }
 
