@@ -234,16 +234,19 @@ typedef struct mhd_hip_batch_stats {
     int active;    /* still stepping (unfinished, not paused) */
     int tiles;     /* wave work items in the current tile list */
     int seg_rows;  /* rows per tile of the current tile list (0: capacity fill) */
-    int blocks;    /* workgroups of the persistent stage kernel */
+    int blocks;    /* workgroups of the largest persistent stage kernel */
     size_t bytes_used, bytes_budget;
     double last_seconds; /* device time of the last mhd_hip_batch_advance */
     int paused;          /* resident, unfinished, excluded from stepping */
 } mhd_hip_batch_stats;
 
-/* HBM bytes one instance of an nx x ny grid occupies in the arena (U and U*). */
+/* HBM bytes one instance of an nx x ny grid occupies in the arena (U and U*; an instance with
+ * solid cells also holds its mask, nx * ny bytes, and 24 bytes per solid cell of wall plan,
+ * each rounded up to 256 bytes). */
 size_t mhd_hip_batch_instance_bytes(int precision, int nx, int ny);
 
-/* physics: limiter and non-ideal fields are used (shared by all instances); others ignored. */
+/* physics: the limiter and non-ideal fields are the scheme of the instances admitted with
+ * mhd_hip_batch_add; others ignored. */
 int mhd_hip_batch_create(int precision, const mhd_hip_params* physics, size_t hbm_budget_bytes,
                          int max_instances, mhd_hip_batch** out);
 void mhd_hip_batch_destroy(mhd_hip_batch* b);
@@ -253,6 +256,20 @@ void mhd_hip_batch_destroy(mhd_hip_batch* b);
  * the instance slots are exhausted. Instances join at the next replay. */
 int mhd_hip_batch_add(mhd_hip_batch* b, const mhd_hip_instance* inst, const double* const comps[9],
                       int* id);
+
+/* Admission with the instance's own scheme: its slope limiter and non-ideal terms are taken
+ * from `scheme` (limiter, ohmic/hall/ambipolar and their coefficients; the other fields of
+ * `scheme` are ignored, `inst` gives them), so one engine advances instances of different
+ * schemes together. `mask` (NULL: all fluid) is the ny x nx interior FluidMask of
+ * include/mhd/mask.hpp:17-83 (1 = fluid, `mask_len` = nx * ny, else EINVAL): the CFL speed is
+ * taken over fluid cells only and the right-hand side is zero on solid cells; with
+ * `reflective_walls` the solid cells take the reflective wall fill (include/mhd/mask.hpp:
+ * 85-149) after the ghost fill before every stage and at the end, as MHDSolver does with
+ * set_walls_reflective(true). The mask and the wall plan are kept in the arena beside the
+ * state. Otherwise as mhd_hip_batch_add. */
+int mhd_hip_batch_add_config(mhd_hip_batch* b, const mhd_hip_instance* inst,
+                             const mhd_hip_params* scheme, const uint8_t* mask, size_t mask_len,
+                             int reflective_walls, const double* const comps[9], int* id);
 
 /* Releases an instance's slot and memory (finished or not). */
 int mhd_hip_batch_remove(mhd_hip_batch* b, int id);

@@ -89,6 +89,9 @@ SIGNATURES = [
                                        C.POINTER(C.c_void_p)]),
     ("mhd_hip_batch_destroy", None, [C.c_void_p]),
     ("mhd_hip_batch_add", C.c_int, [C.c_void_p, C.POINTER(Instance), DP9, C.POINTER(C.c_int)]),
+    ("mhd_hip_batch_add_config", C.c_int, [C.c_void_p, C.POINTER(Instance), C.POINTER(Params),
+                                           C.POINTER(C.c_uint8), C.c_size_t, C.c_int, DP9,
+                                           C.POINTER(C.c_int)]),
     ("mhd_hip_batch_remove", C.c_int, [C.c_void_p, C.c_int]),
     ("mhd_hip_batch_advance", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
     ("mhd_hip_batch_result", C.c_int, [C.c_void_p, C.c_int, C.POINTER(Result),

@@ -8,6 +8,7 @@ depend on which other instances share the batch.
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
 import math
 
 import numpy as np
@@ -43,6 +44,8 @@ class BatchEngine:
                                                 int(max_instances), C.byref(h)))
         self.handle = h
         self.precision = precision
+        self.limiter = SlopeLimiter(limiter)
+        self.nonideal = dataclasses.replace(ni)
         self._shape = {}
         self._ids = (C.c_int * int(max_instances))()
 
@@ -53,9 +56,13 @@ class BatchEngine:
             self.handle = None
 
     def add(self, state, t_end, cfl=0.4, gamma=5.0 / 3.0, glm_alpha=0.1, bc=(0, 0),
-            max_steps=1_000_000, x_min=0.0, x_max=1.0, y_min=0.0, y_max=1.0) -> int:
+            max_steps=1_000_000, x_min=0.0, x_max=1.0, y_min=0.0, y_max=1.0, limiter=None,
+            nonideal=None, mask=None, walls=True) -> int:
         """Admit one solve; `state` is (9, ny+4, nx+4) float64 (ghosts included).
-        Raises CapacityFull when the HBM budget or the slots are exhausted."""
+        `limiter` and `nonideal` default to the engine's; `mask` is an optional (ny, nx) uint8
+        FluidMask (1 = fluid) and `walls` its reflective wall fill, as MHDSolver.set_mask /
+        set_walls_reflective. Raises CapacityFull when the HBM budget or the slots are
+        exhausted."""
         U = np.ascontiguousarray(state, dtype=np.float64)
         ny, nx = U.shape[1] - 4, U.shape[2] - 4
         d = _lib.Instance(nx, ny, x_min, x_max, y_min, y_max, int(bc[0]), int(bc[1]),
@@ -63,7 +70,24 @@ class BatchEngine:
                           int(max_steps))
         ptrs, _ = _lib.planes(U)
         i = C.c_int()
-        _lib.check(self._l.mhd_hip_batch_add(self.handle, C.byref(d), ptrs, C.byref(i)))
+        if limiter is None and nonideal is None and mask is None:
+            _lib.check(self._l.mhd_hip_batch_add(self.handle, C.byref(d), ptrs, C.byref(i)))
+        else:
+            p = _lib.Params()
+            p.limiter = int(self.limiter if limiter is None else limiter)
+            ni = self.nonideal if nonideal is None else nonideal
+            p.ohmic, p.hall, p.ambipolar = int(ni.ohmic), int(ni.hall), int(ni.ambipolar)
+            p.eta_ohm, p.eta_hall, p.eta_ambipolar = ni.eta_ohm, ni.eta_hall, ni.eta_ambipolar
+            if mask is None:
+                mptr, mlen = None, 0
+            else:
+                m = np.ascontiguousarray(mask, dtype=np.uint8)
+                if m.shape != (ny, nx):
+                    raise ValueError("mask shape must be (ny, nx)")
+                mptr, mlen = m.ctypes.data_as(C.POINTER(C.c_uint8)), m.size
+            _lib.check(self._l.mhd_hip_batch_add_config(self.handle, C.byref(d), C.byref(p), mptr,
+                                                        mlen, int(bool(walls)), ptrs,
+                                                        C.byref(i)))
         self._shape[i.value] = (ny + 4, nx + 4)
         return i.value
 

@@ -10,6 +10,7 @@
 #include "mhd_abi.hpp"
 #include "mhd_batch_plan.hpp"
 #include "mhd_hip.h"
+#include "mhd_host.hpp"
 #include "mhd_launch.hpp"
 #include "mhd_transfer.hpp"
 
@@ -35,6 +36,7 @@ using mhdhip::Geom;
 using mhdhip::guarded;
 using mhdhip::InvalidArgument;
 using mhdhip::NoDevice;
+using mhdhip::NonIdeal;
 using mhdhip::Physics;
 
 namespace {
@@ -105,9 +107,15 @@ class Arena {
 struct Slot {
     bool used = false;
     int id = -1;
-    size_t off = 0;    // arena offset of U (U* follows)
-    size_t bytes = 0;  // U + U*
+    size_t off = 0;          // arena offset of U; U*, the mask and the wall plan follow
+    size_t bytes = 0;        // everything the instance holds in the arena
+    size_t state_bytes = 0;  // one state (U or U*), rounded to the arena alignment
     mhd_hip_instance desc{};
+    int limiter = MHD_HIP_MC;
+    NonIdeal ni{};           // the instance's non-ideal terms
+    bool masked = false;     // its mask has solid cells (the mask follows U*)
+    bool walls = true;       // solid cells take the reflective wall fill
+    int plan_count = 0;      // solid cells in the wall plan (follows the mask)
     bool finished = false;
     bool paused = false;  // resident but not stepped (scheduler's choice)
     mhd_hip_result result{};
@@ -119,6 +127,37 @@ size_t state_elems(int nx, int ny)
     return static_cast<size_t>(nx + 4) * static_cast<size_t>(ny + 4);
 }
 
+// include/physics_config/mhd_config.hpp:90-94
+bool nonideal_any(const NonIdeal& n)
+{
+    return (n.ohmic && n.eta_ohm != 0.0) || (n.hall && n.eta_hall != 0.0) ||
+           (n.ambipolar && n.eta_ambipolar != 0.0);
+}
+
+NonIdeal nonideal_of(const mhd_hip_params& p)
+{
+    NonIdeal n{};
+    n.ohmic = p.ohmic;
+    n.hall = p.hall;
+    n.ambipolar = p.ambipolar;
+    n.eta_ohm = p.eta_ohm;
+    n.eta_hall = p.eta_hall;
+    n.eta_ambipolar = p.eta_ambipolar;
+    return n;
+}
+
+// Kernel variant of an instance: limiter (bit 0), non-ideal terms (bit 1), a mask (bit 2:
+// ghosts and, with reflective walls, solid cells are filled in memory before each stage
+// instead of addressed at their sources).
+constexpr int kGroups = 8;
+int group_of(const Slot& s)
+{
+    return (s.masked ? 4 : 0) + (nonideal_any(s.ni) ? 2 : 0) + (s.limiter == MHD_HIP_MC ? 1 : 0);
+}
+int group_limiter(int g) { return (g & 1) ? MHD_HIP_MC : MHD_HIP_MINMOD; }
+bool group_nonideal(int g) { return (g & 2) != 0; }
+bool group_remap(int g) { return (g & 4) == 0; }
+
 }  // namespace
 
 struct mhd_hip_batch {
@@ -127,9 +166,7 @@ struct mhd_hip_batch {
     int device = -1;
     int precision = MHD_HIP_FP64;
     size_t elem = 8;
-    int limiter = MHD_HIP_MC;
-    bool nonideal = false;
-    mhd_hip_params phys_template{};
+    mhd_hip_params phys_template{};  // limiter and non-ideal terms of mhd_hip_batch_add
 
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -142,13 +179,15 @@ struct mhd_hip_batch {
     BatchInst* d_insts = nullptr;
     Ctrl* d_ctrl = nullptr;
     BatchTile* d_tiles = nullptr;
-    int* d_first = nullptr;  // per-wave tile offsets (waves + 1)
+    int* d_first = nullptr;  // per kernel variant, per-wave tile offsets (max_waves + 1 each)
+    int* d_fill = nullptr;   // count, then the slots filled before each stage
     size_t tile_cap = 0;
 
     BatchInst* h_insts = nullptr;  // pinned staging
     Ctrl* h_ctrl = nullptr;
     BatchTile* h_tiles = nullptr;
     int* h_first = nullptr;
+    int* h_fill = nullptr;
     size_t h_tile_cap = 0;
     double* h_stage = nullptr;  // pinned staging for state upload/download
     size_t h_stage_bytes = 0;
@@ -167,9 +206,14 @@ struct mhd_hip_batch {
     int tuned_rows = 0;  // the timed choice: > 0 fixed rows, < 0 fill with -tuned_rows min rows
     int last_rows = 0;  // fixed rows of the current tile list (0: capacity fill)
     int ntiles = 0;
-    int blocks = 0;
+    int blocks = 0;                      // the largest persistent grid of the variants
+    int group_blocks[kGroups] = {};      // persistent grid of each kernel variant
+    int max_waves = 0;
+    unsigned used_groups = 0;            // variants with stepping instances (bit g)
+    bool fill_needed = false;            // stepping instances with a mask
     double last_seconds = 0.0;
     hipGraphExec_t graph = nullptr;
+    unsigned graph_key = 0;              // the variants and fill the captured graph launches
 
     // kernel timing mode: replays are launched eagerly with events around each stage kernel
     bool kernel_timing = false;
@@ -188,10 +232,12 @@ struct mhd_hip_batch {
         if (d_ctrl) (void)hipFree(d_ctrl);
         if (d_tiles) (void)hipFree(d_tiles);
         if (d_first) (void)hipFree(d_first);
+        if (d_fill) (void)hipFree(d_fill);
         if (h_insts) (void)hipHostFree(h_insts);
         if (h_ctrl) (void)hipHostFree(h_ctrl);
         if (h_tiles) (void)hipHostFree(h_tiles);
         if (h_first) (void)hipHostFree(h_first);
+        if (h_fill) (void)hipHostFree(h_fill);
         if (h_stage) (void)hipHostFree(h_stage);
         if (totals_partial) (void)hipFree(totals_partial);
         if (totals_out) (void)hipFree(totals_out);
@@ -225,12 +271,10 @@ struct mhd_hip_batch {
         for (int k = 0; k < kSteps; ++k) {
             const int p = k & 1;
             check(hipEventRecord(tev[4 * k], stream), "hipEventRecord");
-            check(mhdhip::launch_batch_stage<R>(1, limiter, nonideal, tables(), p, blocks, stream),
-                  "batch stage1");
+            launch_stage<R>(1, p);
             check(hipEventRecord(tev[4 * k + 1], stream), "hipEventRecord");
             check(hipEventRecord(tev[4 * k + 2], stream), "hipEventRecord");
-            check(mhdhip::launch_batch_stage<R>(2, limiter, nonideal, tables(), p, blocks, stream),
-                  "batch stage2");
+            launch_stage<R>(2, p);
             check(hipEventRecord(tev[4 * k + 3], stream), "hipEventRecord");
         }
         check(hipMemcpyAsync(h_ctrl, d_ctrl, sizeof(Ctrl) * max_instances, hipMemcpyDeviceToHost,
@@ -280,10 +324,6 @@ struct mhd_hip_batch {
         precision = prec;
         elem = prec == MHD_HIP_FP64 ? 8 : 4;
         phys_template = tmpl;
-        limiter = tmpl.limiter;
-        // include/physics_config/mhd_config.hpp:90-94
-        nonideal = (tmpl.ohmic && tmpl.eta_ohm != 0.0) || (tmpl.hall && tmpl.eta_hall != 0.0) ||
-                   (tmpl.ambipolar && tmpl.eta_ambipolar != 0.0);
         max_instances = max_inst;
         slots.assign(max_inst, Slot{});
         check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -296,26 +336,41 @@ struct mhd_hip_batch {
               "hipMalloc(insts)");
         check(hipMalloc(reinterpret_cast<void**>(&d_ctrl), sizeof(Ctrl) * max_inst),
               "hipMalloc(ctrl)");
-        const int per_cu = precision == MHD_HIP_FP64
-                               ? mhdhip::batch_blocks_per_cu<double>(limiter, nonideal)
-                               : mhdhip::batch_blocks_per_cu<float>(limiter, nonideal);
-        blocks = std::max(1, per_cu) * prop.multiProcessorCount;
-        const size_t nfirst = static_cast<size_t>(waves()) + 1;
+        for (int g = 0; g < kGroups; ++g) {
+            const int per_cu =
+                precision == MHD_HIP_FP64
+                    ? mhdhip::batch_blocks_per_cu<double>(group_limiter(g), group_nonideal(g),
+                                                          group_remap(g))
+                    : mhdhip::batch_blocks_per_cu<float>(group_limiter(g), group_nonideal(g),
+                                                         group_remap(g));
+            group_blocks[g] = std::max(1, per_cu) * prop.multiProcessorCount;
+            blocks = std::max(blocks, group_blocks[g]);
+        }
+        max_waves = blocks * mhdhip::kWavesPerBlock;
+        const size_t nfirst = static_cast<size_t>(kGroups) * (max_waves + 1);
         check(hipMalloc(reinterpret_cast<void**>(&d_first), sizeof(int) * nfirst),
               "hipMalloc(first)");
+        check(hipMalloc(reinterpret_cast<void**>(&d_fill), sizeof(int) * (max_inst + 1)),
+              "hipMalloc(fill)");
         check(hipHostMalloc(reinterpret_cast<void**>(&h_insts), sizeof(BatchInst) * max_inst, 0),
               "hipHostMalloc");
         check(hipHostMalloc(reinterpret_cast<void**>(&h_ctrl), sizeof(Ctrl) * max_inst, 0),
               "hipHostMalloc");
         check(hipHostMalloc(reinterpret_cast<void**>(&h_first), sizeof(int) * nfirst, 0),
               "hipHostMalloc");
+        check(hipHostMalloc(reinterpret_cast<void**>(&h_fill), sizeof(int) * (max_inst + 1), 0),
+              "hipHostMalloc");
         std::memset(h_insts, 0, sizeof(BatchInst) * max_inst);
         std::memset(h_ctrl, 0, sizeof(Ctrl) * max_inst);
         std::fill(h_first, h_first + nfirst, 0);
+        std::fill(h_fill, h_fill + max_inst + 1, 0);
         check(hipMemcpyAsync(d_insts, h_insts, sizeof(BatchInst) * max_inst,
                              hipMemcpyHostToDevice, stream),
               "hipMemcpyAsync");
         check(hipMemcpyAsync(d_first, h_first, sizeof(int) * nfirst, hipMemcpyHostToDevice,
+                             stream),
+              "hipMemcpyAsync");
+        check(hipMemcpyAsync(d_fill, h_fill, sizeof(int) * (max_inst + 1), hipMemcpyHostToDevice,
                              stream),
               "hipMemcpyAsync");
         ensure_tiles(1024);
@@ -358,10 +413,22 @@ struct mhd_hip_batch {
         h_stage_bytes = bytes;
     }
 
-    static size_t instance_bytes(int prec, int nx, int ny)
+    static size_t state_bytes_of(int prec, int nx, int ny)
     {
         const size_t e = prec == MHD_HIP_FP64 ? 8 : 4;
-        return 2 * round_up(9 * state_elems(nx, ny) * e, kAlign);
+        return round_up(9 * state_elems(nx, ny) * e, kAlign);
+    }
+
+    // Arena bytes of an instance: U and U*, and with solid cells the mask and the wall plan.
+    static size_t instance_bytes(int prec, int nx, int ny, bool masked = false,
+                                 size_t plan_count = 0)
+    {
+        size_t b = 2 * state_bytes_of(prec, nx, ny);
+        if (masked) {
+            b += round_up(static_cast<size_t>(nx) * ny, kAlign) +
+                 round_up(3 * sizeof(long long) * plan_count, kAlign);
+        }
+        return b;
     }
 
     Slot& slot_of(int id)
@@ -376,8 +443,25 @@ struct mhd_hip_batch {
 
     int slot_index(const Slot& s) const { return static_cast<int>(&s - slots.data()); }
 
-    Physics physics_of(const mhd_hip_instance& d) const
+    char* base_of(const Slot& s) const { return static_cast<char*>(arena_base) + s.off; }
+    const uint8_t* mask_of(const Slot& s) const
     {
+        return s.masked ? reinterpret_cast<const uint8_t*>(base_of(s) + 2 * s.state_bytes)
+                        : nullptr;
+    }
+    const long long* plan_of(const Slot& s) const
+    {
+        if (!s.masked) return nullptr;
+        const size_t mask_bytes = round_up(static_cast<size_t>(s.desc.nx) * s.desc.ny, kAlign);
+        return reinterpret_cast<const long long*>(base_of(s) + 2 * s.state_bytes + mask_bytes);
+    }
+
+    // The instance's Physics: gamma, CFL number, spacings, end time and step limit from its
+    // description, the non-ideal terms and their diffusive dt bound from its own settings
+    // (include/physics_config/mhd_config.hpp:96-110, include/mhd/cfl.hpp:77-89).
+    static Physics physics_of(const Slot& s)
+    {
+        const mhd_hip_instance& d = s.desc;
         const double dx = (d.x_max - d.x_min) / static_cast<double>(d.nx);
         const double dy = (d.y_max - d.y_min) / static_cast<double>(d.ny);
         Physics ph{};
@@ -391,7 +475,7 @@ struct mhd_hip_batch {
         ph.inv_2dy = 0.5 / dy;
         ph.glm_alpha = d.glm_alpha;
         ph.h = (dy < dx) ? dy : dx;
-        const mhd_hip_params& t = phys_template;
+        const NonIdeal& t = s.ni;
         double eta = 0.0;
         auto mx = [](double a, double b) { return (a < b) ? b : a; };
         if (t.ohmic) eta = mx(eta, std::fabs(t.eta_ohm));
@@ -401,12 +485,7 @@ struct mhd_hip_batch {
         ph.dt_diff = ph.has_diff ? d.cfl * ph.h * ph.h / (2.0 * 2.0 * eta) : 0.0;
         ph.t_end = d.t_end;
         ph.max_steps = d.max_steps;
-        ph.ni.ohmic = t.ohmic;
-        ph.ni.hall = t.hall;
-        ph.ni.ambipolar = t.ambipolar;
-        ph.ni.eta_ohm = t.eta_ohm;
-        ph.ni.eta_hall = t.eta_hall;
-        ph.ni.eta_ambipolar = t.eta_ambipolar;
+        ph.ni = t;
         return ph;
     }
 
@@ -432,8 +511,7 @@ struct mhd_hip_batch {
         ensure_stage(9 * n * sizeof(R));
         R* st = reinterpret_cast<R*>(h_stage);
         mhdhip::gather_planes(st, comps, 9, n);
-        char* base = static_cast<char*>(arena_base) + s.off;
-        check(hipMemcpyAsync(base, st, 9 * n * sizeof(R), hipMemcpyHostToDevice, stream),
+        check(hipMemcpyAsync(base_of(s), st, 9 * n * sizeof(R), hipMemcpyHostToDevice, stream),
               "hipMemcpyAsync(upload)");
     }
 
@@ -442,10 +520,11 @@ struct mhd_hip_batch {
     {
         const Slot& s = slots[si];
         const Geom g = geom_of(s.desc);
-        const Physics ph = physics_of(s.desc);
-        R* u = reinterpret_cast<R*>(static_cast<char*>(arena_base) + s.off);
+        const Physics ph = physics_of(s);
+        R* u = reinterpret_cast<R*>(base_of(s));
         check(mhdhip::launch_ctrl_init<R>(d_ctrl + si, stream), "ctrl_init");
-        check(mhdhip::launch_cfl<R>(u, g, ph, nullptr, d_ctrl + si, stream), "cfl");
+        // include/mhd/cfl.hpp:46-71: the fluid interior cells only
+        check(mhdhip::launch_cfl<R>(u, g, ph, mask_of(s), d_ctrl + si, stream), "cfl");
         // the host copy of this slot's control block, so its progress reads right before the
         // first replay
         check(hipMemcpyAsync(h_ctrl + si, d_ctrl + si, sizeof(Ctrl), hipMemcpyDeviceToHost,
@@ -454,7 +533,11 @@ struct mhd_hip_batch {
         check(hipStreamSynchronize(stream), "hipStreamSynchronize");
     }
 
-    int add(const mhd_hip_instance& d, const double* const comps[9])
+    // Admission of one instance with its own limiter and non-ideal terms (`scheme`), an
+    // optional interior mask (null: all fluid; include/mhd/mask.hpp:17-83) and the reflective
+    // wall fill of its solid cells (walls; include/mhd/mask.hpp:85-149).
+    int add(const mhd_hip_instance& d, const mhd_hip_params& scheme, const uint8_t* mask,
+            size_t mask_len, bool walls, const double* const comps[9])
     {
         if (d.nx < 1 || d.ny < 1) {
             throw InvalidArgument("instance grid must have at least one interior cell per axis");
@@ -472,6 +555,24 @@ struct mhd_hip_batch {
         if (!valid_bc(d.bc_x) || !valid_bc(d.bc_y)) {
             throw InvalidArgument("instance boundaries must be periodic or outflow");
         }
+        if (scheme.limiter != MHD_HIP_MINMOD && scheme.limiter != MHD_HIP_MC) {
+            throw InvalidArgument("unknown slope limiter");
+        }
+        const size_t cells = static_cast<size_t>(d.nx) * d.ny;
+        if (mask != nullptr && mask_len != cells) {
+            throw InvalidArgument("FluidMask size does not match Grid2D interior");
+        }
+        // solver_api.hpp:300: an all-fluid mask is no mask
+        bool masked = false;
+        for (size_t k = 0; mask != nullptr && k < cells && !masked; ++k) {
+            masked = mask[k] == 0;
+        }
+        std::vector<long long> plan;
+        if (masked && walls) {
+            const mhd_hip_grid grid{d.nx, d.ny, 2, d.x_min, d.x_max, d.y_min, d.y_max};
+            mhdhip::wall_plan(grid, mask, plan);
+        }
+        const size_t plan_count = plan.size() / 3;
         int si = -1;
         for (int k = 0; k < max_instances; ++k) {
             if (!slots[k].used) {
@@ -482,7 +583,7 @@ struct mhd_hip_batch {
         if (si < 0) {
             throw Full("all instance slots are in use");
         }
-        const size_t bytes = instance_bytes(precision, d.nx, d.ny);
+        const size_t bytes = instance_bytes(precision, d.nx, d.ny, masked, plan_count);
         size_t off = 0;
         if (!arena.alloc(bytes, off)) {
             throw Full("HBM budget exhausted (" + std::to_string(arena.used()) + " of " +
@@ -495,8 +596,26 @@ struct mhd_hip_batch {
         s.id = next_id++;
         s.off = off;
         s.bytes = bytes;
+        s.state_bytes = state_bytes_of(precision, d.nx, d.ny);
         s.desc = d;
+        s.limiter = scheme.limiter;
+        s.ni = nonideal_of(scheme);
+        s.masked = masked;
+        s.walls = walls;
+        s.plan_count = static_cast<int>(plan_count);
         try {
+            if (masked) {
+                // mask and plan from host memory that lives until start_instance synchronises
+                check(hipMemcpyAsync(const_cast<uint8_t*>(mask_of(s)), mask, cells,
+                                     hipMemcpyHostToDevice, stream),
+                      "hipMemcpyAsync(mask)");
+                if (plan_count > 0) {
+                    check(hipMemcpyAsync(const_cast<long long*>(plan_of(s)), plan.data(),
+                                         plan.size() * sizeof(long long), hipMemcpyHostToDevice,
+                                         stream),
+                          "hipMemcpyAsync(plan)");
+                }
+            }
             if (precision == MHD_HIP_FP64) {
                 upload_state<double>(s, comps);
                 start_instance<double>(si);
@@ -505,6 +624,7 @@ struct mhd_hip_batch {
                 start_instance<float>(si);
             }
         } catch (...) {
+            (void)hipStreamSynchronize(stream);
             arena.release(off, bytes);
             s = Slot{};
             throw;
@@ -530,27 +650,28 @@ struct mhd_hip_batch {
     static constexpr int kTimedRowCandidates[] = {8, 16, 24, 32, 48, 64};
     static constexpr int kFillMinRows[] = {32};
 
-    int waves() const { return blocks * mhdhip::kWavesPerBlock; }
+    int group_waves(int g) const { return group_blocks[g] * mhdhip::kWavesPerBlock; }
 
-    std::vector<mhdhip::PlanInst> stepping_set() const
+    // The stepping instances of kernel variant g.
+    std::vector<mhdhip::PlanInst> stepping_set(int g) const
     {
         std::vector<mhdhip::PlanInst> insts;
         for (int si = 0; si < max_instances; ++si) {
             const Slot& s = slots[si];
-            if (s.used && !s.finished && !s.paused) {
+            if (s.used && !s.finished && !s.paused && group_of(s) == g) {
                 insts.push_back(mhdhip::PlanInst{si, s.desc.nx, s.desc.ny});
             }
         }
         return insts;
     }
 
-    // The partition a mode stands for on the current stepping set: > 0 fixed segments of that
-    // many rows handed out round-robin (0: the load model's length for this set); < 0 the
-    // capacity fill with pieces of at least -mode rows.
-    int resolve(int mode, const std::vector<mhdhip::PlanInst>& insts) const
+    // The partition a mode stands for on a stepping set: > 0 fixed segments of that many rows
+    // handed out round-robin (0: the load model's length for the set); < 0 the capacity fill
+    // with pieces of at least -mode rows.
+    static int resolve(int mode, const std::vector<mhdhip::PlanInst>& insts, int waves)
     {
         if (mode != 0) return mode;
-        return mhdhip::model_rows(insts, waves(),
+        return mhdhip::model_rows(insts, waves,
                                   std::vector<int>(std::begin(kModelRowCandidates),
                                                    std::end(kModelRowCandidates)));
     }
@@ -562,14 +683,32 @@ struct mhd_hip_batch {
         return 0;
     }
 
+    // Tables of the stepping set: every slot's BatchInst; per kernel variant, its tiles and
+    // per-wave offsets (variant g's offsets at g * (max_waves + 1)); the list of instances
+    // whose ghosts and solid cells are filled before each stage.
     void build_tables(int mode)
     {
-        const std::vector<mhdhip::PlanInst> insts = stepping_set();
-        const int r = resolve(mode, insts);
-        const mhdhip::WavePlan p = r < 0 ? mhdhip::plan_fill(insts, waves(), -r)
-                                         : mhdhip::plan_fixed(insts, r, waves());
-        last_rows = r > 0 ? r : 0;
-        const size_t n = p.tiles.size();
+        std::vector<BatchTile> tiles;
+        std::vector<int> first(static_cast<size_t>(kGroups) * (max_waves + 1), 0);
+        used_groups = 0;
+        last_rows = 0;
+        for (int g = 0; g < kGroups; ++g) {
+            const std::vector<mhdhip::PlanInst> insts = stepping_set(g);
+            const int base = static_cast<int>(tiles.size());
+            int* f = first.data() + static_cast<size_t>(g) * (max_waves + 1);
+            std::fill(f, f + max_waves + 1, base);
+            if (insts.empty()) continue;
+            const int waves = group_waves(g);
+            const int r = resolve(mode, insts, waves);
+            const mhdhip::WavePlan p = r < 0 ? mhdhip::plan_fill(insts, waves, -r)
+                                             : mhdhip::plan_fixed(insts, r, waves);
+            for (int w = 0; w <= waves; ++w) f[w] = base + p.first[static_cast<size_t>(w)];
+            tiles.insert(tiles.end(), p.tiles.begin(), p.tiles.end());
+            used_groups |= 1u << g;
+            if (last_rows == 0) last_rows = r > 0 ? r : 0;
+        }
+        int nfill = 0;
+        const size_t n = tiles.size();
         ensure_tiles(std::max<size_t>(n, 1));
         check(hipStreamSynchronize(stream), "hipStreamSynchronize");  // staging reuse
         for (int si = 0; si < max_instances; ++si) {
@@ -579,16 +718,22 @@ struct mhd_hip_batch {
             if (!s.used) {
                 continue;
             }
-            char* base = static_cast<char*>(arena_base) + s.off;
-            in.U = base;
-            in.Us = base + s.bytes / 2;
+            in.U = base_of(s);
+            in.Us = base_of(s) + s.state_bytes;
+            in.mask = mask_of(s);
+            in.plan = s.walls ? plan_of(s) : nullptr;
+            in.plan_count = s.walls ? s.plan_count : 0;
             in.g = geom_of(s.desc);
-            in.ph = physics_of(s.desc);
+            in.ph = physics_of(s);
             in.enabled = 1;
+            if (s.masked && !s.finished && !s.paused) {
+                h_fill[1 + nfill++] = si;
+            }
         }
-        std::copy(p.tiles.begin(), p.tiles.end(), h_tiles);
-        const size_t nfirst = static_cast<size_t>(waves()) + 1;
-        std::copy(p.first.begin(), p.first.end(), h_first);
+        h_fill[0] = nfill;
+        fill_needed = nfill > 0;
+        std::copy(tiles.begin(), tiles.end(), h_tiles);
+        std::copy(first.begin(), first.end(), h_first);
         ntiles = static_cast<int>(n);
         check(hipMemcpyAsync(d_insts, h_insts, sizeof(BatchInst) * max_instances,
                              hipMemcpyHostToDevice, stream),
@@ -598,30 +743,60 @@ struct mhd_hip_batch {
                                  stream),
                   "hipMemcpyAsync(tiles)");
         }
-        check(hipMemcpyAsync(d_first, h_first, sizeof(int) * nfirst, hipMemcpyHostToDevice,
+        check(hipMemcpyAsync(d_first, h_first, sizeof(int) * first.size(), hipMemcpyHostToDevice,
                              stream),
               "hipMemcpyAsync(first)");
+        check(hipMemcpyAsync(d_fill, h_fill, sizeof(int) * (nfill + 1), hipMemcpyHostToDevice,
+                             stream),
+              "hipMemcpyAsync(fill)");
     }
 
-    BatchTables tables() const { return BatchTables{d_insts, d_ctrl, d_tiles, d_first}; }
+    BatchTables tables(int g) const
+    {
+        return BatchTables{d_insts, d_ctrl, d_tiles, d_first + static_cast<size_t>(g) * (max_waves + 1),
+                           d_fill, d_fill + 1};
+    }
 
+    // One RK stage of the stepping set: the ghost and wall fill of the masked instances
+    // (include/mhd/time_integrator.hpp:46-56), then one persistent launch per kernel variant.
+    template <typename R>
+    void launch_stage(int stage, int parity)
+    {
+        if (fill_needed) {
+            check(mhdhip::launch_batch_fill<R>(tables(0), parity, stage, max_instances, stream),
+                  "batch fill");
+        }
+        for (int g = 0; g < kGroups; ++g) {
+            if (used_groups & (1u << g)) {
+                check(mhdhip::launch_batch_stage<R>(stage, group_limiter(g), group_nonideal(g),
+                                                    group_remap(g), tables(g), parity,
+                                                    group_blocks[g], stream),
+                      stage == 1 ? "batch stage1" : "batch stage2");
+            }
+        }
+    }
+
+    unsigned launch_key() const { return used_groups | (fill_needed ? 1u << kGroups : 0u); }
+
+    // The captured graph of kSteps steps; captured again when the kernel variants in use (or
+    // the need of the fill) change, since the graph holds one launch per variant.
     template <typename R>
     void ensure_graph()
     {
-        if (graph) {
+        if (graph && graph_key == launch_key()) {
             return;
+        }
+        if (graph) {
+            (void)hipGraphExecDestroy(graph);
+            graph = nullptr;
         }
         hipGraph_t g = nullptr;
         check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "BeginCapture");
         try {
             for (int k = 0; k < kSteps; ++k) {
                 const int p = k & 1;
-                check(mhdhip::launch_batch_stage<R>(1, limiter, nonideal, tables(), p, blocks,
-                                                    stream),
-                      "batch stage1");
-                check(mhdhip::launch_batch_stage<R>(2, limiter, nonideal, tables(), p, blocks,
-                                                    stream),
-                      "batch stage2");
+                launch_stage<R>(1, p);
+                launch_stage<R>(2, p);
             }
         } catch (...) {
             (void)hipStreamEndCapture(stream, &g);
@@ -632,11 +807,12 @@ struct mhd_hip_batch {
         const hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         check(e, "hipGraphInstantiate");
+        graph_key = launch_key();
     }
 
-    // Timed choice (rows per wave -1): times one stage-1 pass (which leaves U untouched) over
-    // the current instance set for the capacity fill and each fixed row length, and keeps the
-    // fastest partition.
+    // Timed choice (rows per wave -1): times one stage-1 pass (which leaves U's interior
+    // untouched) over the current instance set for the capacity fill and each fixed row
+    // length, and keeps the fastest partition.
     template <typename R>
     void autotune()
     {
@@ -647,14 +823,11 @@ struct mhd_hip_batch {
         float best_ms = 1e30f;
         for (const int rows : modes) {
             build_tables(rows);
-            check(mhdhip::launch_batch_stage<R>(1, limiter, nonideal, tables(), 0, blocks, stream),
-                  "tune");
+            launch_stage<R>(1, 0);
             float ms_best = 1e30f;
             for (int rep = 0; rep < 2; ++rep) {
                 check(hipEventRecord(ev0, stream), "hipEventRecord");
-                check(mhdhip::launch_batch_stage<R>(1, limiter, nonideal, tables(), 0, blocks,
-                                                    stream),
-                      "tune");
+                launch_stage<R>(1, 0);
                 check(hipEventRecord(ev1, stream), "hipEventRecord");
                 check(hipEventSynchronize(ev1), "hipEventSynchronize");
                 float ms = 0.0f;
@@ -701,7 +874,7 @@ struct mhd_hip_batch {
         if (c.steps[0] > 0) {
             return c.t[0] / c.steps[0];
         }
-        const Physics ph = physics_of(s.desc);
+        const Physics ph = physics_of(s);
         double ch;
         const unsigned long long bits = c.chmax[0];
         std::memcpy(&ch, &bits, sizeof(ch));
@@ -840,14 +1013,27 @@ struct mhd_hip_batch {
                                          : advance_t<float>(max_replays);
     }
 
+    // include/mhd/time_integrator.hpp:101-104: the final ghost fill of a solve, and the wall
+    // fill of its solid cells
+    template <typename R>
+    void final_fill(const Slot& s)
+    {
+        const Geom g = geom_of(s.desc);
+        R* u = reinterpret_cast<R*>(base_of(s));
+        const Gate open{d_ctrl, 0.0, 0, -1};
+        check(mhdhip::launch_ghost_fill<R>(u, g, open, stream), "ghost_fill");
+        if (s.masked && s.walls) {
+            check(mhdhip::launch_wall_fill<R>(u, g, plan_of(s), s.plan_count, s.desc.gamma, open,
+                                              stream),
+                  "wall_fill");
+        }
+    }
+
     template <typename R>
     void download_t(const Slot& s, double* const comps[9])
     {
-        const Geom g = geom_of(s.desc);
-        R* u = reinterpret_cast<R*>(static_cast<char*>(arena_base) + s.off);
-        const Gate open{d_ctrl, 0.0, 0, -1};
-        // include/mhd/time_integrator.hpp:101-102: the final ghost fill of a solve
-        check(mhdhip::launch_ghost_fill<R>(u, g, open, stream), "ghost_fill");
+        R* u = reinterpret_cast<R*>(base_of(s));
+        final_fill<R>(s);
         const size_t n = state_elems(s.desc.nx, s.desc.ny);
         ensure_stage(9 * n * sizeof(R));
         check(hipMemcpyAsync(h_stage, u, 9 * n * sizeof(R), hipMemcpyDeviceToHost, stream),
@@ -923,10 +1109,9 @@ struct mhd_hip_batch {
     double max_div_b_t(const Slot& s)
     {
         const Geom g = geom_of(s.desc);
-        const Physics ph = physics_of(s.desc);
-        R* u = reinterpret_cast<R*>(static_cast<char*>(arena_base) + s.off);
-        const Gate open{d_ctrl, 0.0, 0, -1};
-        check(mhdhip::launch_ghost_fill<R>(u, g, open, stream), "ghost_fill");
+        const Physics ph = physics_of(s);
+        R* u = reinterpret_cast<R*>(base_of(s));
+        final_fill<R>(s);
         auto* out = reinterpret_cast<unsigned long long*>(totals_out);
         check(mhdhip::launch_max_div_b<R>(u, g, ph.inv_2dx, ph.inv_2dy, out, stream), "max_div_b");
         double v = 0.0;
@@ -997,7 +1182,20 @@ int mhd_hip_batch_add(mhd_hip_batch* b, const mhd_hip_instance* inst,
         for (int c = 0; c < 9; ++c) {
             if (!comps[c]) throw InvalidArgument("null component pointer");
         }
-        *id = b->add(*inst, comps);
+        *id = b->add(*inst, b->phys_template, nullptr, 0, true, comps);
+    });
+}
+
+int mhd_hip_batch_add_config(mhd_hip_batch* b, const mhd_hip_instance* inst,
+                             const mhd_hip_params* scheme, const uint8_t* mask, size_t mask_len,
+                             int reflective_walls, const double* const comps[9], int* id)
+{
+    return guarded([&] {
+        if (!b || !inst || !scheme || !comps || !id) throw InvalidArgument("null argument");
+        for (int c = 0; c < 9; ++c) {
+            if (!comps[c]) throw InvalidArgument("null component pointer");
+        }
+        *id = b->add(*inst, *scheme, mask, mask_len, reflective_walls != 0, comps);
     });
 }
 

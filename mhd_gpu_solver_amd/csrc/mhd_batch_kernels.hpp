@@ -18,19 +18,23 @@ static_assert(kPlanStripCells == kStripCells && kPlanHalfCells == kHalfCells,
               "the host planner and the kernels must agree on the strip widths");
 
 struct BatchInst {
-    void* U;       // this instance's 9 planes of U
-    void* Us;      // and of U*
+    void* U;                // this instance's 9 planes of U
+    void* Us;               // and of U*
+    const uint8_t* mask;    // interior mask (1 = fluid), null when all cells are fluid
+    const long long* plan;  // solid-cell fill plan (dst, src or -1, axis), reflective walls
     Geom g;
     Physics ph;
+    int plan_count;
     int enabled;
-    int pad;
 };
 
 struct BatchTables {
     const BatchInst* insts;
     Ctrl* ctrl;  // one control block per instance slot
     const BatchTile* tiles;
-    const int* first;  // per-wave offsets into tiles: one more than the waves of the launch
+    const int* first;       // this launch's per-wave offsets into tiles (waves + 1)
+    const int* fill_count;  // how many slots fill_list holds
+    const int* fill_list;   // masked slots, whose ghosts and solid cells are filled per stage
 };
 
 // Waves per SIMD each batched stage kernel is compiled for: what its registers allow (fp32
@@ -39,7 +43,12 @@ struct BatchTables {
 template <typename R, int NI>
 constexpr int kBatchWavesPerEU = sizeof(R) == 4 ? (NI ? 3 : 4) : (NI ? 1 : 2);
 
-template <int STAGE, int LIM, int NI, typename R>
+// REMAP 1: instances without solid cells; ghosts are addressed at their interior sources.
+// REMAP 0: instances with a mask; ghosts (and, with reflective walls, the solid cells) are
+// read from memory, filled by k_batch_fill before the stage as include/mhd/
+// time_integrator.hpp:46-56 fills before every stage, and the mask zeroes the solid cells'
+// right-hand side and keeps them out of the CFL speed.
+template <int STAGE, int LIM, int NI, int REMAP, typename R>
 __global__ __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(kBatchWavesPerEU<R, NI>)))
 void k_batch_stage(BatchTables tb, int parity)
@@ -78,8 +87,8 @@ void k_batch_stage(BatchTables tb, int parity)
         const R* u0 = static_cast<const R*>(in.U);
         R* dst = static_cast<R*>(STAGE == 1 ? in.Us : in.U);
         const int dB = tile.r0b < 0 ? 0 : tile.r0b - tile.r0;
-        const R local = march_tile<STAGE, LIM, NI, 1, 2, 1, R>(
-            src, u0, dst, nullptr, in.g, in.ph, st, tile.strip, tile.r0, tile.r1, dB,
+        const R local = march_tile<STAGE, LIM, NI, REMAP, 2, 1, R>(
+            src, u0, dst, REMAP ? nullptr : in.mask, in.g, in.ph, st, tile.strip, tile.r0, tile.r1, dB,
             tile.r0b < 0 ? tile.r1 : tile.r1b, ring[wave], lane);
         if (STAGE == 2) {
             const R m = wave_max(local);
@@ -87,6 +96,32 @@ void k_batch_stage(BatchTables tb, int parity)
                 atomic_max_speed(&ctrl->chmax[parity ^ 1], static_cast<double>(m));
             }
         }
+    }
+}
+
+// Ghost fill, then solid-wall fill, of the stage input of every listed instance whose step
+// with this parity is still pending: one workgroup per instance, so the ghosts take the solid
+// cells' values before the wall fill replaces them, as the separate fills do in order
+// (include/boundary_conditions/boundary_conditions.hpp:33-103, include/mhd/mask.hpp:85-149).
+template <typename R>
+__global__ __launch_bounds__(256) void k_batch_fill(BatchTables tb, int parity, int stage)
+{
+    if (static_cast<int>(blockIdx.x) >= *tb.fill_count) {
+        return;
+    }
+    const int si = tb.fill_list[blockIdx.x];
+    const BatchInst& in = tb.insts[si];
+    if (!in.enabled || !ctrl_pending(tb.ctrl + si, in.ph, parity)) {
+        return;
+    }
+    R* U = static_cast<R*>(stage == 1 ? in.U : in.Us);
+    const long long total = ghost_cells(in.g);
+    for (long long t = threadIdx.x; t < total; t += blockDim.x) {
+        ghost_fill_cell(U, in.g, t);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < in.plan_count; k += blockDim.x) {
+        wall_fill_cell(U, in.g, in.plan, k, in.ph.gamma);
     }
 }
 

@@ -241,54 +241,64 @@ __device__ __forceinline__ bool gate_open(const Gate& gt)
     return gt.ctrl->t[gt.parity] < gt.t_end && gt.ctrl->steps[gt.parity] < gt.max_steps;
 }
 
+// Ghost cell t of the 2 ng (nxt + ny) ghost cells of a grid, in the order bottom band, top
+// band, then the left and right columns of the interior rows, takes its interior source.
+template <typename R>
+__device__ __forceinline__ void ghost_fill_cell(R* __restrict__ U, const Geom& g, long long t)
+{
+    const long long band = static_cast<long long>(g.ng) * g.nxt;
+    int i, j;
+    if (t < band) {
+        j = static_cast<int>(t / g.nxt);
+        i = static_cast<int>(t - static_cast<long long>(j) * g.nxt);
+    } else if (t < 2 * band) {
+        const long long t2 = t - band;
+        j = g.ng + g.ny + static_cast<int>(t2 / g.nxt);
+        i = static_cast<int>(t2 % g.nxt);
+    } else {
+        const long long t3 = t - 2 * band;
+        const int w = 2 * g.ng;
+        j = g.ng + static_cast<int>(t3 / w);
+        const int k = static_cast<int>(t3 % w);
+        i = k < g.ng ? k : g.nx + k;
+    }
+    const int si = ghost_source(i, g.nx, g.ng, g.bc_x);
+    const int sj = ghost_source(j, g.ny, g.ng, g.bc_y);
+    const long long dst = static_cast<long long>(j) * g.nxt + i;
+    const long long src = static_cast<long long>(sj) * g.nxt + si;
+#pragma unroll
+    for (int c = 0; c < kNC; ++c) {
+        U[c * g.plane + dst] = U[c * g.plane + src];
+    }
+}
+
+__device__ __host__ __forceinline__ long long ghost_cells(const Geom& g)
+{
+    return 2 * static_cast<long long>(g.ng) * g.nxt + static_cast<long long>(g.ny) * 2 * g.ng;
+}
+
 template <typename R>
 __global__ __launch_bounds__(256) void k_ghost_fill(R* __restrict__ U, Geom g, Gate gt)
 {
     if (!gate_open(gt)) {
         return;
     }
-    const long long band = static_cast<long long>(g.ng) * g.nxt;
-    const long long total = 2 * band + static_cast<long long>(g.ny) * 2 * g.ng;
+    const long long total = ghost_cells(g);
     for (long long t = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; t < total;
          t += static_cast<long long>(gridDim.x) * blockDim.x) {
-        int i, j;
-        if (t < band) {
-            j = static_cast<int>(t / g.nxt);
-            i = static_cast<int>(t - static_cast<long long>(j) * g.nxt);
-        } else if (t < 2 * band) {
-            const long long t2 = t - band;
-            j = g.ng + g.ny + static_cast<int>(t2 / g.nxt);
-            i = static_cast<int>(t2 % g.nxt);
-        } else {
-            const long long t3 = t - 2 * band;
-            const int w = 2 * g.ng;
-            j = g.ng + static_cast<int>(t3 / w);
-            const int k = static_cast<int>(t3 % w);
-            i = k < g.ng ? k : g.nx + k;
-        }
-        const int si = ghost_source(i, g.nx, g.ng, g.bc_x);
-        const int sj = ghost_source(j, g.ny, g.ng, g.bc_y);
-        const long long dst = static_cast<long long>(j) * g.nxt + i;
-        const long long src = static_cast<long long>(sj) * g.nxt + si;
-#pragma unroll
-        for (int c = 0; c < kNC; ++c) {
-            U[c * g.plane + dst] = U[c * g.plane + src];
-        }
+        ghost_fill_cell(U, g, t);
     }
 }
 
 // ---- solid wall fill: include/mhd/mask.hpp:85-149 ------------------------------------------
 // plan[3k] = destination cell, plan[3k+1] = fluid source cell or -1, plan[3k+2] = axis (0 x, 1 y)
 // Sources are always fluid cells, so every entry is independent of the others.
+// Solid cell k of the plan takes its reflected fluid neighbour (or the quiescent default).
 template <typename R>
-__global__ __launch_bounds__(256) void k_wall_fill(R* __restrict__ U, Geom g,
-                                                   const long long* __restrict__ plan, int count,
-                                                   double gamma, Gate gt)
+__device__ __forceinline__ void wall_fill_cell(R* __restrict__ U, const Geom& g,
+                                               const long long* __restrict__ plan, int k,
+                                               double gamma)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= count || !gate_open(gt)) {
-        return;
-    }
     const long long dst = plan[3 * k];
     const long long src = plan[3 * k + 1];
     const int axis = static_cast<int>(plan[3 * k + 2]);
@@ -347,6 +357,18 @@ __global__ __launch_bounds__(256) void k_wall_fill(R* __restrict__ U, Geom g,
     for (int c = 0; c < kNC; ++c) {
         U[c * g.plane + dst] = static_cast<R>(O[c]);
     }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_wall_fill(R* __restrict__ U, Geom g,
+                                                   const long long* __restrict__ plan, int count,
+                                                   double gamma, Gate gt)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count || !gate_open(gt)) {
+        return;
+    }
+    wall_fill_cell(U, g, plan, k, gamma);
 }
 
 // ---- compensated totals of the conserved fields -----------------------------------------

@@ -73,13 +73,20 @@ hipError_t launch_max_div_b(const R* U, const Geom& g, double inv_2dx, double in
 template <typename R>
 int stage_blocks_per_cu(int limiter, bool nonideal, bool remap);
 
-// Batched engine: one persistent launch of `blocks` workgroups per stage.
+// Batched engine: one persistent launch of `blocks` workgroups per stage and kernel variant
+// (remap: ghosts addressed at their sources; otherwise read from memory after k_batch_fill).
 template <typename R>
-hipError_t launch_batch_stage(int stage, int limiter, bool nonideal, const BatchTables& tb,
-                              int parity, int blocks, hipStream_t s);
+hipError_t launch_batch_stage(int stage, int limiter, bool nonideal, bool remap,
+                              const BatchTables& tb, int parity, int blocks, hipStream_t s);
+
+// Ghost and solid-wall fill of the listed instances' stage input, one workgroup per slot of
+// `slots` (the engine's slot count; workgroups past the list's length return at once).
+template <typename R>
+hipError_t launch_batch_fill(const BatchTables& tb, int parity, int stage, int slots,
+                             hipStream_t s);
 
 // Workgroups of k_batch_stage that fit on one CU at once (occupancy query).
 template <typename R>
-int batch_blocks_per_cu(int limiter, bool nonideal);
+int batch_blocks_per_cu(int limiter, bool nonideal, bool remap);
 
 }  // namespace mhdhip

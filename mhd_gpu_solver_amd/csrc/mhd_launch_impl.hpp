@@ -101,50 +101,74 @@ hipError_t launch_stage(int stage, int limiter, bool nonideal, bool remap, const
 namespace detail {
 
 template <int STAGE, int LIM, int NI, typename R>
-hipError_t launch_batch_one(const BatchTables& tb, int parity, int blocks, hipStream_t s)
+hipError_t launch_batch_one(bool remap, const BatchTables& tb, int parity, int blocks,
+                            hipStream_t s)
 {
-    hipLaunchKernelGGL((k_batch_stage<STAGE, LIM, NI, R>), dim3(blocks), dim3(kBlock), 0, s, tb,
-                       parity);
+    if (remap) {
+        hipLaunchKernelGGL((k_batch_stage<STAGE, LIM, NI, 1, R>), dim3(blocks), dim3(kBlock), 0, s,
+                           tb, parity);
+    } else {
+        hipLaunchKernelGGL((k_batch_stage<STAGE, LIM, NI, 0, R>), dim3(blocks), dim3(kBlock), 0, s,
+                           tb, parity);
+    }
     return hipGetLastError();
 }
 
 template <int STAGE, typename R>
-hipError_t launch_batch_lim(int limiter, bool ni, const BatchTables& tb, int parity, int blocks,
-                            hipStream_t s)
+hipError_t launch_batch_lim(int limiter, bool ni, bool remap, const BatchTables& tb, int parity,
+                            int blocks, hipStream_t s)
 {
     if (limiter == kMinmod) {
-        return ni ? launch_batch_one<STAGE, kMinmod, 1, R>(tb, parity, blocks, s)
-                  : launch_batch_one<STAGE, kMinmod, 0, R>(tb, parity, blocks, s);
+        return ni ? launch_batch_one<STAGE, kMinmod, 1, R>(remap, tb, parity, blocks, s)
+                  : launch_batch_one<STAGE, kMinmod, 0, R>(remap, tb, parity, blocks, s);
     }
-    return ni ? launch_batch_one<STAGE, kMC, 1, R>(tb, parity, blocks, s)
-              : launch_batch_one<STAGE, kMC, 0, R>(tb, parity, blocks, s);
+    return ni ? launch_batch_one<STAGE, kMC, 1, R>(remap, tb, parity, blocks, s)
+              : launch_batch_one<STAGE, kMC, 0, R>(remap, tb, parity, blocks, s);
 }
 
 }  // namespace detail
 
 template <typename R>
-hipError_t launch_batch_stage(int stage, int limiter, bool nonideal, const BatchTables& tb,
-                              int parity, int blocks, hipStream_t s)
+hipError_t launch_batch_stage(int stage, int limiter, bool nonideal, bool remap,
+                              const BatchTables& tb, int parity, int blocks, hipStream_t s)
 {
-    return stage == 1 ? detail::launch_batch_lim<1, R>(limiter, nonideal, tb, parity, blocks, s)
-                      : detail::launch_batch_lim<2, R>(limiter, nonideal, tb, parity, blocks, s);
+    return stage == 1
+               ? detail::launch_batch_lim<1, R>(limiter, nonideal, remap, tb, parity, blocks, s)
+               : detail::launch_batch_lim<2, R>(limiter, nonideal, remap, tb, parity, blocks, s);
 }
 
 template <typename R>
-int batch_blocks_per_cu(int limiter, bool nonideal)
+hipError_t launch_batch_fill(const BatchTables& tb, int parity, int stage, int slots,
+                             hipStream_t s)
+{
+    hipLaunchKernelGGL((k_batch_fill<R>), dim3(slots), dim3(256), 0, s, tb, parity, stage);
+    return hipGetLastError();
+}
+
+namespace detail {
+
+template <int LIM, int NI, typename R>
+hipError_t batch_occupancy(bool remap, int* n)
+{
+    return remap ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                       n, k_batch_stage<2, LIM, NI, 1, R>, kBlock, 0)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                       n, k_batch_stage<2, LIM, NI, 0, R>, kBlock, 0);
+}
+
+}  // namespace detail
+
+template <typename R>
+int batch_blocks_per_cu(int limiter, bool nonideal, bool remap)
 {
     int n = 0;
     hipError_t e;
     if (limiter == kMinmod) {
-        e = nonideal ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                           &n, k_batch_stage<2, kMinmod, 1, R>, kBlock, 0)
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                           &n, k_batch_stage<2, kMinmod, 0, R>, kBlock, 0);
+        e = nonideal ? detail::batch_occupancy<kMinmod, 1, R>(remap, &n)
+                     : detail::batch_occupancy<kMinmod, 0, R>(remap, &n);
     } else {
-        e = nonideal ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                           &n, k_batch_stage<2, kMC, 1, R>, kBlock, 0)
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                           &n, k_batch_stage<2, kMC, 0, R>, kBlock, 0);
+        e = nonideal ? detail::batch_occupancy<kMC, 1, R>(remap, &n)
+                     : detail::batch_occupancy<kMC, 0, R>(remap, &n);
     }
     return (e == hipSuccess && n > 0) ? n : 1;
 }
@@ -214,9 +238,10 @@ int stage_blocks_per_cu(int limiter, bool nonideal, bool remap)
                                             const Gate&, hipStream_t);                           \
     template hipError_t launch_stage<R>(int, int, bool, bool, const StageBuffers&, const Geom&,  \
                                         const Physics&, const Tiling&, int, hipStream_t);         \
-    template hipError_t launch_batch_stage<R>(int, int, bool, const BatchTables&, int, int,       \
+    template hipError_t launch_batch_stage<R>(int, int, bool, bool, const BatchTables&, int, int, \
                                               hipStream_t);                                       \
-    template int batch_blocks_per_cu<R>(int, bool);                                              \
+    template hipError_t launch_batch_fill<R>(const BatchTables&, int, int, int, hipStream_t);      \
+    template int batch_blocks_per_cu<R>(int, bool, bool);                                        \
     template int stage_blocks_per_cu<R>(int, bool, bool);                                        \
     template hipError_t launch_totals<R>(const R*, const Geom&, DD*, double*, hipStream_t);     \
     template hipError_t launch_max_div_b<R>(const R*, const Geom&, double, double,              \

@@ -33,6 +33,17 @@ from .batch import BatchEngine, CapacityFull, instance_bytes
 from .solver import SlopeLimiter
 
 
+def _scheme(req) -> dict:
+    """A request's own limiter, non-ideal terms and mask, where it names them."""
+    out = {}
+    for key in ("limiter", "nonideal", "mask"):
+        if getattr(req, key, None) is not None:
+            out[key] = getattr(req, key)
+    if "mask" in out:
+        out["walls"] = getattr(req, "walls", True)
+    return out
+
+
 @dataclasses.dataclass
 class Completed:
     index: int
@@ -153,8 +164,12 @@ class Server:
                 try:
                     iid = self.engine.add(fut.result(), req.t_end, cfl=self.cfl,
                                           gamma=req.gamma, glm_alpha=self.glm_alpha, bc=req.bc,
-                                          max_steps=req.max_steps)
+                                          max_steps=req.max_steps, **_scheme(req))
                 except CapacityFull:
+                    if not active:
+                        # nothing resident and still no room: it can never be admitted
+                        raise ValueError(f"request {k} ({req.n}^2) exceeds the whole HBM "
+                                         "budget") from None
                     break
                 queue.popleft()
                 del states[k]

@@ -29,6 +29,10 @@ class Spec:
     seed: int = 0
     arrival: float = 0.0
     max_steps: int = 1_000_000
+    limiter: object = None   # SlopeLimiter; None: the engine's
+    nonideal: object = None  # NonIdealConfig; None: the engine's
+    mask: object = None      # (n, n) uint8 FluidMask (1 = fluid); None: all fluid
+    walls: bool = True       # reflective wall fill of the mask's solid cells
 
     def state(self):
         return init_ic(self.ic, self.n, self.n, gamma=self.gamma, b0=self.b0, seed=self.seed)
@@ -89,3 +93,14 @@ def circles_mask(n, count=8, seed=11, r_min=0.02, r_max=0.06):
         cx, cy = rng.uniform(r, 1.0 - r, size=2)
         mask[(X - cx) ** 2 + (Y - cy) ** 2 <= r * r] = 0
     return mask
+
+
+def rotor_circles(n, seed=11, t_end=math.inf, eta_ohm=1e-3, eta_hall=1e-2, eta_ambipolar=1e-3):
+    """The non-ideal + geometry solve as one request: the rotor with Ohmic, Hall and ambipolar
+    terms, outflow boundaries, minmod, and a mask of 8 seeded solid circles acting as
+    reflective walls."""
+    from .solver import NonIdealConfig, SlopeLimiter
+    ni = NonIdealConfig(ohmic=True, hall=True, ambipolar=True, eta_ohm=eta_ohm,
+                        eta_hall=eta_hall, eta_ambipolar=eta_ambipolar)
+    return Spec("rotor", n, 5.0 / 3.0, 1.0, (OUTFLOW, OUTFLOW), t_end, limiter=SlopeLimiter.Minmod,
+                nonideal=ni, mask=circles_mask(n, 8, seed=seed), walls=True)

@@ -182,3 +182,11 @@ def test_request_larger_than_budget_rejected():
     eng = FakeEngine()
     with pytest.raises(ValueError):
         make_server(eng, budget=1024).run([Req(0, replays=1, n=256)], realtime=False)
+
+
+def test_request_that_never_fits_an_empty_engine_is_rejected():
+    """A request the size check passes but the empty engine still cannot hold (e.g. its mask
+    and wall plan on top of the state) is an error, not an endless wait."""
+    eng = FakeEngine(slots=0)
+    with pytest.raises(ValueError):
+        make_server(eng).run([Req(0, replays=1)], realtime=False)
