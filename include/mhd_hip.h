@@ -37,7 +37,13 @@ enum mhd_hip_status {
 };
 
 /* mhd::Precision (device_solver.hpp:28) */
-enum mhd_hip_precision { MHD_HIP_FP64 = 0, MHD_HIP_FP32 = 1 };
+/* MHD_HIP_FP64: fp64, bit-identical to the reference's CPU path. MHD_HIP_FP32: FP32 storage
+ * and arithmetic with fp64 pressure recovery and fast-speed discriminant. MHD_HIP_FP64_FAST
+ * (no counterpart in the reference's Precision): fp64 storage and arithmetic with contracted
+ * multiply-adds and Newton-refined hardware reciprocal and square root in the stepping
+ * kernels; within 1e-12 relative L1 of MHD_HIP_FP64 at the same dt sequence, not bit-equal.
+ * mhd_hip_precision reports it as created. */
+enum mhd_hip_precision { MHD_HIP_FP64 = 0, MHD_HIP_FP32 = 1, MHD_HIP_FP64_FAST = 2 };
 
 /* BoundaryConditionType (include/boundary_conditions/boundary_conditions.hpp:8) */
 enum mhd_hip_bc { MHD_HIP_PERIODIC = 0, MHD_HIP_REFLECTIVE = 1, MHD_HIP_OUTFLOW = 2 };

@@ -13,7 +13,8 @@ PKG = pathlib.Path(__file__).resolve().parent
 LIB_PATH = PKG / "libmhd_hip.so"
 
 OK, EINVAL, ERUNTIME, ENODEV, EFULL = 0, 1, 2, 3, 4
-FP64, FP32 = 0, 1
+FP64, FP32, FP64_FAST = 0, 1, 2
+PRECISIONS = {"fp64": FP64, "fp32": FP32, "fp64_fast": FP64_FAST}
 FILL_GHOSTS, FILL_WALLS = 1, 2
 
 

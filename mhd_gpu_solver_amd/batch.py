@@ -25,9 +25,9 @@ def instance_bytes(precision: str, nx: int, ny: int) -> int:
 
 
 def _prec(precision):
-    if precision not in ("fp64", "fp32"):
-        raise ValueError("precision must be 'fp64' or 'fp32'")
-    return _lib.FP64 if precision == "fp64" else _lib.FP32
+    if precision not in _lib.PRECISIONS:
+        raise ValueError("precision must be 'fp64', 'fp32' or 'fp64_fast'")
+    return _lib.PRECISIONS[precision]
 
 
 class BatchEngine:

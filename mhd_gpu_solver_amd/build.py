@@ -6,6 +6,7 @@ file, so that it travels with the source tree and is the object the package load
 * mhd_fp64.hip, mhd_solver.hip, mhd_host.cpp, ... : -ffp-contract=off (fp64 parity with the
   CPU path depends on every product and sum rounding separately)
 * mhd_fp32.hip (single-run and batched FP32 kernels): contraction allowed (FP32-bulk path)
+* mhd_fp64fast.hip (the fast fp64 stepping kernels): contraction allowed
 * all HIP code: packed fp32 instructions disabled (-packed-fp32-ops)
 """
 from __future__ import annotations
@@ -30,7 +31,7 @@ COMMON = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
-CONTRACT_FAST = {"mhd_fp32.hip"}
+CONTRACT_FAST = {"mhd_fp32.hip", "mhd_fp64fast.hip"}
 
 
 def hipcc() -> str:

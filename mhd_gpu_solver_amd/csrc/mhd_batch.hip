@@ -252,7 +252,7 @@ struct mhd_hip_batch {
     // kernel. After the synchronisation the control blocks tell how many of the kSteps steps
     // stepped some instance (the later launches found every instance finished and returned at
     // once): only those launches and their times are added.
-    template <typename R>
+    template <typename R, int FAST>
     void timed_replay()
     {
         std::vector<int> before(static_cast<size_t>(max_instances), 0);
@@ -271,10 +271,10 @@ struct mhd_hip_batch {
         for (int k = 0; k < kSteps; ++k) {
             const int p = k & 1;
             check(hipEventRecord(tev[4 * k], stream), "hipEventRecord");
-            launch_stage<R>(1, p);
+            launch_stage<R, FAST>(1, p);
             check(hipEventRecord(tev[4 * k + 1], stream), "hipEventRecord");
             check(hipEventRecord(tev[4 * k + 2], stream), "hipEventRecord");
-            launch_stage<R>(2, p);
+            launch_stage<R, FAST>(2, p);
             check(hipEventRecord(tev[4 * k + 3], stream), "hipEventRecord");
         }
         check(hipMemcpyAsync(h_ctrl, d_ctrl, sizeof(Ctrl) * max_instances, hipMemcpyDeviceToHost,
@@ -299,8 +299,9 @@ struct mhd_hip_batch {
 
     void init(int prec, const mhd_hip_params& tmpl, size_t budget, int max_inst)
     {
-        if (prec != MHD_HIP_FP64 && prec != MHD_HIP_FP32) {
-            throw InvalidArgument("precision must be MHD_HIP_FP64 or MHD_HIP_FP32");
+        if (prec != MHD_HIP_FP64 && prec != MHD_HIP_FP32 && prec != MHD_HIP_FP64_FAST) {
+            throw InvalidArgument(
+                "precision must be MHD_HIP_FP64, MHD_HIP_FP32 or MHD_HIP_FP64_FAST");
         }
         if (tmpl.limiter != MHD_HIP_MINMOD && tmpl.limiter != MHD_HIP_MC) {
             throw InvalidArgument("unknown slope limiter");
@@ -322,7 +323,7 @@ struct mhd_hip_batch {
             throw NoDevice("current HIP device is not gfx950 (MI355X)");
         }
         precision = prec;
-        elem = prec == MHD_HIP_FP64 ? 8 : 4;
+        elem = prec == MHD_HIP_FP32 ? 4 : 8;
         phys_template = tmpl;
         max_instances = max_inst;
         slots.assign(max_inst, Slot{});
@@ -337,12 +338,14 @@ struct mhd_hip_batch {
         check(hipMalloc(reinterpret_cast<void**>(&d_ctrl), sizeof(Ctrl) * max_inst),
               "hipMalloc(ctrl)");
         for (int g = 0; g < kGroups; ++g) {
+            const int lim = group_limiter(g);
+            const bool ni = group_nonideal(g), remap = group_remap(g);
             const int per_cu =
                 precision == MHD_HIP_FP64
-                    ? mhdhip::batch_blocks_per_cu<double>(group_limiter(g), group_nonideal(g),
-                                                          group_remap(g))
-                    : mhdhip::batch_blocks_per_cu<float>(group_limiter(g), group_nonideal(g),
-                                                         group_remap(g));
+                    ? mhdhip::batch_blocks_per_cu<double, 0>(lim, ni, remap)
+                    : precision == MHD_HIP_FP64_FAST
+                          ? mhdhip::batch_blocks_per_cu<double, 1>(lim, ni, remap)
+                          : mhdhip::batch_blocks_per_cu<float, 0>(lim, ni, remap);
             group_blocks[g] = std::max(1, per_cu) * prop.multiProcessorCount;
             blocks = std::max(blocks, group_blocks[g]);
         }
@@ -415,7 +418,7 @@ struct mhd_hip_batch {
 
     static size_t state_bytes_of(int prec, int nx, int ny)
     {
-        const size_t e = prec == MHD_HIP_FP64 ? 8 : 4;
+        const size_t e = prec == MHD_HIP_FP32 ? 4 : 8;
         return round_up(9 * state_elems(nx, ny) * e, kAlign);
     }
 
@@ -515,7 +518,7 @@ struct mhd_hip_batch {
               "hipMemcpyAsync(upload)");
     }
 
-    template <typename R>
+    template <typename R, int FAST>
     void start_instance(int si)
     {
         const Slot& s = slots[si];
@@ -524,7 +527,7 @@ struct mhd_hip_batch {
         R* u = reinterpret_cast<R*>(base_of(s));
         check(mhdhip::launch_ctrl_init<R>(d_ctrl + si, stream), "ctrl_init");
         // include/mhd/cfl.hpp:46-71: the fluid interior cells only
-        check(mhdhip::launch_cfl<R>(u, g, ph, mask_of(s), d_ctrl + si, stream), "cfl");
+        check(mhdhip::launch_cfl<R, FAST>(u, g, ph, mask_of(s), d_ctrl + si, stream), "cfl");
         // the host copy of this slot's control block, so its progress reads right before the
         // first replay
         check(hipMemcpyAsync(h_ctrl + si, d_ctrl + si, sizeof(Ctrl), hipMemcpyDeviceToHost,
@@ -618,10 +621,13 @@ struct mhd_hip_batch {
             }
             if (precision == MHD_HIP_FP64) {
                 upload_state<double>(s, comps);
-                start_instance<double>(si);
+                start_instance<double, 0>(si);
+            } else if (precision == MHD_HIP_FP64_FAST) {
+                upload_state<double>(s, comps);
+                start_instance<double, 1>(si);
             } else {
                 upload_state<float>(s, comps);
-                start_instance<float>(si);
+                start_instance<float, 0>(si);
             }
         } catch (...) {
             (void)hipStreamSynchronize(stream);
@@ -759,7 +765,7 @@ struct mhd_hip_batch {
 
     // One RK stage of the stepping set: the ghost and wall fill of the masked instances
     // (include/mhd/time_integrator.hpp:46-56), then one persistent launch per kernel variant.
-    template <typename R>
+    template <typename R, int FAST>
     void launch_stage(int stage, int parity)
     {
         if (fill_needed) {
@@ -768,7 +774,7 @@ struct mhd_hip_batch {
         }
         for (int g = 0; g < kGroups; ++g) {
             if (used_groups & (1u << g)) {
-                check(mhdhip::launch_batch_stage<R>(stage, group_limiter(g), group_nonideal(g),
+                check(mhdhip::launch_batch_stage<R, FAST>(stage, group_limiter(g), group_nonideal(g),
                                                     group_remap(g), tables(g), parity,
                                                     group_blocks[g], stream),
                       stage == 1 ? "batch stage1" : "batch stage2");
@@ -780,7 +786,7 @@ struct mhd_hip_batch {
 
     // The captured graph of kSteps steps; captured again when the kernel variants in use (or
     // the need of the fill) change, since the graph holds one launch per variant.
-    template <typename R>
+    template <typename R, int FAST>
     void ensure_graph()
     {
         if (graph && graph_key == launch_key()) {
@@ -795,8 +801,8 @@ struct mhd_hip_batch {
         try {
             for (int k = 0; k < kSteps; ++k) {
                 const int p = k & 1;
-                launch_stage<R>(1, p);
-                launch_stage<R>(2, p);
+                launch_stage<R, FAST>(1, p);
+                launch_stage<R, FAST>(2, p);
             }
         } catch (...) {
             (void)hipStreamEndCapture(stream, &g);
@@ -813,7 +819,7 @@ struct mhd_hip_batch {
     // Timed choice (rows per wave -1): times one stage-1 pass (which leaves U's interior
     // untouched) over the current instance set for the capacity fill and each fixed row
     // length, and keeps the fastest partition.
-    template <typename R>
+    template <typename R, int FAST>
     void autotune()
     {
         std::vector<int> modes;
@@ -823,11 +829,11 @@ struct mhd_hip_batch {
         float best_ms = 1e30f;
         for (const int rows : modes) {
             build_tables(rows);
-            launch_stage<R>(1, 0);
+            launch_stage<R, FAST>(1, 0);
             float ms_best = 1e30f;
             for (int rep = 0; rep < 2; ++rep) {
                 check(hipEventRecord(ev0, stream), "hipEventRecord");
-                launch_stage<R>(1, 0);
+                launch_stage<R, FAST>(1, 0);
                 check(hipEventRecord(ev1, stream), "hipEventRecord");
                 check(hipEventSynchronize(ev1), "hipEventSynchronize");
                 float ms = 0.0f;
@@ -959,7 +965,7 @@ struct mhd_hip_batch {
         }
     }
 
-    template <typename R>
+    template <typename R, int FAST>
     int advance_t(int max_replays)
     {
         check(hipSetDevice(device), "hipSetDevice");
@@ -979,7 +985,7 @@ struct mhd_hip_batch {
                 break;
             }
             if (seg_rows == -1 && tuned_rows == 0) {
-                autotune<R>();
+                autotune<R, FAST>();
             }
             if (dirty) {
                 build_tables(plan_mode());
@@ -987,9 +993,9 @@ struct mhd_hip_batch {
             }
             note_launch(last);
             if (kernel_timing) {
-                timed_replay<R>();
+                timed_replay<R, FAST>();
             } else {
-                ensure_graph<R>();
+                ensure_graph<R, FAST>();
                 check(hipGraphLaunch(graph, stream), "hipGraphLaunch");
             }
         }
@@ -1009,8 +1015,11 @@ struct mhd_hip_batch {
 
     int advance(int max_replays)
     {
-        return precision == MHD_HIP_FP64 ? advance_t<double>(max_replays)
-                                         : advance_t<float>(max_replays);
+        switch (precision) {
+            case MHD_HIP_FP64: return advance_t<double, 0>(max_replays);
+            case MHD_HIP_FP64_FAST: return advance_t<double, 1>(max_replays);
+            default: return advance_t<float, 0>(max_replays);
+        }
     }
 
     // include/mhd/time_integrator.hpp:101-104: the final ghost fill of a solve, and the wall
@@ -1068,8 +1077,8 @@ struct mhd_hip_batch {
         check(hipStreamSynchronize(ref.stream), "hipStreamSynchronize");
         const void* a = static_cast<const char*>(arena_base) + s.off;
         const void* b = static_cast<const char*>(ref.arena_base) + r.off;
-        check(mhdhip::launch_rel_l1_any(a, precision == MHD_HIP_FP64, b,
-                                        ref.precision == MHD_HIP_FP64, geom_of(s.desc),
+        check(mhdhip::launch_rel_l1_any(a, precision != MHD_HIP_FP32, b,
+                                        ref.precision != MHD_HIP_FP32, geom_of(s.desc),
                                         l1_partial, totals_out, stream),
               "rel_l1");
         check(hipMemcpyAsync(out, totals_out, 9 * sizeof(double), hipMemcpyDeviceToHost, stream),
@@ -1090,7 +1099,7 @@ struct mhd_hip_batch {
         }
         const Geom g = geom_of(s.desc);
         void* u = static_cast<char*>(arena_base) + s.off;
-        if (precision == MHD_HIP_FP64) {
+        if (precision != MHD_HIP_FP32) {
             check(mhdhip::launch_totals<double>(static_cast<double*>(u), g, totals_partial,
                                                 totals_out, stream),
                   "totals");
@@ -1129,7 +1138,7 @@ struct mhd_hip_batch {
             check(hipMalloc(reinterpret_cast<void**>(&totals_out), 9 * sizeof(double)),
                   "hipMalloc(totals)");
         }
-        return precision == MHD_HIP_FP64 ? max_div_b_t<double>(s) : max_div_b_t<float>(s);
+        return precision != MHD_HIP_FP32 ? max_div_b_t<double>(s) : max_div_b_t<float>(s);
     }
 
     void download(int id, double* const comps[9], size_t n)
@@ -1139,7 +1148,7 @@ struct mhd_hip_batch {
             throw InvalidArgument("download: size does not match the instance's full layout");
         }
         check(hipSetDevice(device), "hipSetDevice");
-        if (precision == MHD_HIP_FP64) {
+        if (precision != MHD_HIP_FP32) {
             download_t<double>(s, comps);
         } else {
             download_t<float>(s, comps);
@@ -1151,7 +1160,8 @@ extern "C" {
 
 size_t mhd_hip_batch_instance_bytes(int precision, int nx, int ny)
 {
-    if (nx < 1 || ny < 1 || (precision != MHD_HIP_FP64 && precision != MHD_HIP_FP32)) {
+    if (nx < 1 || ny < 1 ||
+        (precision != MHD_HIP_FP64 && precision != MHD_HIP_FP32 && precision != MHD_HIP_FP64_FAST)) {
         return 0;
     }
     return mhd_hip_batch::instance_bytes(precision, nx, ny);

@@ -48,11 +48,12 @@ constexpr int kBatchWavesPerEU = sizeof(R) == 4 ? (NI ? 3 : 4) : (NI ? 1 : 2);
 // read from memory, filled by k_batch_fill before the stage as include/mhd/
 // time_integrator.hpp:46-56 fills before every stage, and the mask zeroes the solid cells'
 // right-hand side and keeps them out of the CFL speed.
-template <int STAGE, int LIM, int NI, int REMAP, typename R>
+template <int STAGE, int LIM, int NI, int REMAP, typename R, int FAST = 0>
 __global__ __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(kBatchWavesPerEU<R, NI>)))
 void k_batch_stage(BatchTables tb, int parity)
 {
+    static_assert(FAST == MHD_FAST_FP64, "stepping kernel built in the wrong translation unit");
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = wave_in_block();
     const int gw = xcd_logical_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;

@@ -193,10 +193,13 @@ __global__ void k_ctrl_init(Ctrl* ctrl)
 }
 
 // ---- CFL over the fluid interior: include/mhd/cfl.hpp:46-71 --------------------------------
-template <typename R>
+// FAST (every stepping kernel): 1 in the fast fp64 translation unit only, so that its kernels,
+// built with other arithmetic, never share a symbol with the exact ones.
+template <typename R, int FAST = 0>
 __global__ __launch_bounds__(256) void k_cfl(const R* __restrict__ U, Geom g, Physics ph,
                                              const uint8_t* __restrict__ mask, Ctrl* ctrl)
 {
+    static_assert(FAST == MHD_FAST_FP64, "stepping kernel built in the wrong translation unit");
     const long long ncell = static_cast<long long>(g.nx) * g.ny;
     R local = R(0);
     const R gamma = static_cast<R>(ph.gamma);
@@ -844,10 +847,11 @@ __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, 
     return local_speed;
 }
 
-template <int STAGE, int LIM, int NI, int REMAP, typename R>
+template <int STAGE, int LIM, int NI, int REMAP, typename R, int FAST = 0>
 __global__ __launch_bounds__(kBlock) void k_stage(StageBuffers b, Geom g, Physics ph, Tiling tl,
                                                   int parity)
 {
+    static_assert(FAST == MHD_FAST_FP64, "stepping kernel built in the wrong translation unit");
     const StepState st = step_state(b.ctrl, ph, parity);
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     if (STAGE == 1 && lead) {

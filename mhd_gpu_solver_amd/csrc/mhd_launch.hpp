@@ -15,7 +15,9 @@ int default_seg_rows(int nx, int ny);
 template <typename R>
 hipError_t launch_ctrl_init(Ctrl* ctrl, hipStream_t s);
 
-template <typename R>
+// FAST = 1: the stepping kernels of the fast fp64 build (R = double), defined in
+// mhd_fp64fast.hip; FAST = 0 everywhere else.
+template <typename R, int FAST = 0>
 hipError_t launch_cfl(const R* U, const Geom& g, const Physics& ph, const uint8_t* mask, Ctrl* ctrl,
                       hipStream_t s);
 
@@ -26,7 +28,7 @@ template <typename R>
 hipError_t launch_wall_fill(R* U, const Geom& g, const long long* plan, int count, double gamma,
                             const Gate& gt, hipStream_t s);
 
-template <typename R>
+template <typename R, int FAST = 0>
 hipError_t launch_stage(int stage, int limiter, bool nonideal, bool remap, const StageBuffers& b,
                         const Geom& g, const Physics& ph, const Tiling& tl, int parity,
                         hipStream_t s);
@@ -70,12 +72,12 @@ hipError_t launch_max_div_b(const R* U, const Geom& g, double inv_2dx, double in
                             unsigned long long* out, hipStream_t s);
 
 // Workgroups of k_stage (stage 2, the larger) that fit on one CU at once (occupancy query).
-template <typename R>
+template <typename R, int FAST = 0>
 int stage_blocks_per_cu(int limiter, bool nonideal, bool remap);
 
 // Batched engine: one persistent launch of `blocks` workgroups per stage and kernel variant
 // (remap: ghosts addressed at their sources; otherwise read from memory after k_batch_fill).
-template <typename R>
+template <typename R, int FAST = 0>
 hipError_t launch_batch_stage(int stage, int limiter, bool nonideal, bool remap,
                               const BatchTables& tb, int parity, int blocks, hipStream_t s);
 
@@ -86,7 +88,7 @@ hipError_t launch_batch_fill(const BatchTables& tb, int parity, int stage, int s
                              hipStream_t s);
 
 // Workgroups of k_batch_stage that fit on one CU at once (occupancy query).
-template <typename R>
+template <typename R, int FAST = 0>
 int batch_blocks_per_cu(int limiter, bool nonideal, bool remap);
 
 }  // namespace mhdhip

@@ -22,25 +22,25 @@ inline unsigned flat_blocks(long long work, int block, int cap)
     return static_cast<unsigned>(std::max<long long>(1, std::min<long long>(b, cap)));
 }
 
-template <int STAGE, int LIM, int NI, int REMAP, typename R>
+template <int STAGE, int LIM, int NI, int REMAP, typename R, int FAST>
 hipError_t launch_one(const StageBuffers& b, const Geom& g, const Physics& ph, const Tiling& tl,
                       int parity, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_stage<STAGE, LIM, NI, REMAP, R>), stage_grid(tl), dim3(kBlock), 0, s, b,
+    hipLaunchKernelGGL((k_stage<STAGE, LIM, NI, REMAP, R, FAST>), stage_grid(tl), dim3(kBlock), 0, s, b,
                        g, ph, tl, parity);
     return hipGetLastError();
 }
 
-template <int STAGE, int LIM, typename R>
+template <int STAGE, int LIM, typename R, int FAST>
 hipError_t launch_ni_remap(bool ni, bool remap, const StageBuffers& b, const Geom& g,
                            const Physics& ph, const Tiling& tl, int parity, hipStream_t s)
 {
     if (ni) {
-        return remap ? launch_one<STAGE, LIM, 1, 1, R>(b, g, ph, tl, parity, s)
-                     : launch_one<STAGE, LIM, 1, 0, R>(b, g, ph, tl, parity, s);
+        return remap ? launch_one<STAGE, LIM, 1, 1, R, FAST>(b, g, ph, tl, parity, s)
+                     : launch_one<STAGE, LIM, 1, 0, R, FAST>(b, g, ph, tl, parity, s);
     }
-    return remap ? launch_one<STAGE, LIM, 0, 1, R>(b, g, ph, tl, parity, s)
-                 : launch_one<STAGE, LIM, 0, 0, R>(b, g, ph, tl, parity, s);
+    return remap ? launch_one<STAGE, LIM, 0, 1, R, FAST>(b, g, ph, tl, parity, s)
+                 : launch_one<STAGE, LIM, 0, 0, R, FAST>(b, g, ph, tl, parity, s);
 }
 
 }  // namespace detail
@@ -52,12 +52,12 @@ hipError_t launch_ctrl_init(Ctrl* ctrl, hipStream_t s)
     return hipGetLastError();
 }
 
-template <typename R>
+template <typename R, int FAST>
 hipError_t launch_cfl(const R* U, const Geom& g, const Physics& ph, const uint8_t* mask, Ctrl* ctrl,
                       hipStream_t s)
 {
     const long long ncell = static_cast<long long>(g.nx) * g.ny;
-    hipLaunchKernelGGL((k_cfl<R>), dim3(detail::flat_blocks(ncell, 256, 4096)), dim3(256), 0, s, U,
+    hipLaunchKernelGGL((k_cfl<R, FAST>), dim3(detail::flat_blocks(ncell, 256, 4096)), dim3(256), 0, s, U,
                        g, ph, mask, ctrl);
     return hipGetLastError();
 }
@@ -83,58 +83,63 @@ hipError_t launch_wall_fill(R* U, const Geom& g, const long long* plan, int coun
     return hipGetLastError();
 }
 
-template <typename R>
+template <typename R, int FAST>
 hipError_t launch_stage(int stage, int limiter, bool nonideal, bool remap, const StageBuffers& b,
                         const Geom& g, const Physics& ph, const Tiling& tl, int parity,
                         hipStream_t s)
 {
     if (stage == 1) {
         return limiter == kMinmod
-                   ? detail::launch_ni_remap<1, kMinmod, R>(nonideal, remap, b, g, ph, tl, parity, s)
-                   : detail::launch_ni_remap<1, kMC, R>(nonideal, remap, b, g, ph, tl, parity, s);
+                   ? detail::launch_ni_remap<1, kMinmod, R, FAST>(nonideal, remap, b, g, ph, tl,
+                                                                  parity, s)
+                   : detail::launch_ni_remap<1, kMC, R, FAST>(nonideal, remap, b, g, ph, tl, parity,
+                                                              s);
     }
     return limiter == kMinmod
-               ? detail::launch_ni_remap<2, kMinmod, R>(nonideal, remap, b, g, ph, tl, parity, s)
-               : detail::launch_ni_remap<2, kMC, R>(nonideal, remap, b, g, ph, tl, parity, s);
+               ? detail::launch_ni_remap<2, kMinmod, R, FAST>(nonideal, remap, b, g, ph, tl,
+                                                              parity, s)
+               : detail::launch_ni_remap<2, kMC, R, FAST>(nonideal, remap, b, g, ph, tl, parity, s);
 }
 
 namespace detail {
 
-template <int STAGE, int LIM, int NI, typename R>
+template <int STAGE, int LIM, int NI, typename R, int FAST>
 hipError_t launch_batch_one(bool remap, const BatchTables& tb, int parity, int blocks,
                             hipStream_t s)
 {
     if (remap) {
-        hipLaunchKernelGGL((k_batch_stage<STAGE, LIM, NI, 1, R>), dim3(blocks), dim3(kBlock), 0, s,
+        hipLaunchKernelGGL((k_batch_stage<STAGE, LIM, NI, 1, R, FAST>), dim3(blocks), dim3(kBlock), 0, s,
                            tb, parity);
     } else {
-        hipLaunchKernelGGL((k_batch_stage<STAGE, LIM, NI, 0, R>), dim3(blocks), dim3(kBlock), 0, s,
+        hipLaunchKernelGGL((k_batch_stage<STAGE, LIM, NI, 0, R, FAST>), dim3(blocks), dim3(kBlock), 0, s,
                            tb, parity);
     }
     return hipGetLastError();
 }
 
-template <int STAGE, typename R>
+template <int STAGE, typename R, int FAST>
 hipError_t launch_batch_lim(int limiter, bool ni, bool remap, const BatchTables& tb, int parity,
                             int blocks, hipStream_t s)
 {
     if (limiter == kMinmod) {
-        return ni ? launch_batch_one<STAGE, kMinmod, 1, R>(remap, tb, parity, blocks, s)
-                  : launch_batch_one<STAGE, kMinmod, 0, R>(remap, tb, parity, blocks, s);
+        return ni ? launch_batch_one<STAGE, kMinmod, 1, R, FAST>(remap, tb, parity, blocks, s)
+                  : launch_batch_one<STAGE, kMinmod, 0, R, FAST>(remap, tb, parity, blocks, s);
     }
-    return ni ? launch_batch_one<STAGE, kMC, 1, R>(remap, tb, parity, blocks, s)
-              : launch_batch_one<STAGE, kMC, 0, R>(remap, tb, parity, blocks, s);
+    return ni ? launch_batch_one<STAGE, kMC, 1, R, FAST>(remap, tb, parity, blocks, s)
+              : launch_batch_one<STAGE, kMC, 0, R, FAST>(remap, tb, parity, blocks, s);
 }
 
 }  // namespace detail
 
-template <typename R>
+template <typename R, int FAST>
 hipError_t launch_batch_stage(int stage, int limiter, bool nonideal, bool remap,
                               const BatchTables& tb, int parity, int blocks, hipStream_t s)
 {
     return stage == 1
-               ? detail::launch_batch_lim<1, R>(limiter, nonideal, remap, tb, parity, blocks, s)
-               : detail::launch_batch_lim<2, R>(limiter, nonideal, remap, tb, parity, blocks, s);
+               ? detail::launch_batch_lim<1, R, FAST>(limiter, nonideal, remap, tb, parity, blocks,
+                                                      s)
+               : detail::launch_batch_lim<2, R, FAST>(limiter, nonideal, remap, tb, parity, blocks,
+                                                      s);
 }
 
 template <typename R>
@@ -147,28 +152,28 @@ hipError_t launch_batch_fill(const BatchTables& tb, int parity, int stage, int s
 
 namespace detail {
 
-template <int LIM, int NI, typename R>
+template <int LIM, int NI, typename R, int FAST>
 hipError_t batch_occupancy(bool remap, int* n)
 {
     return remap ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                       n, k_batch_stage<2, LIM, NI, 1, R>, kBlock, 0)
+                       n, k_batch_stage<2, LIM, NI, 1, R, FAST>, kBlock, 0)
                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                       n, k_batch_stage<2, LIM, NI, 0, R>, kBlock, 0);
+                       n, k_batch_stage<2, LIM, NI, 0, R, FAST>, kBlock, 0);
 }
 
 }  // namespace detail
 
-template <typename R>
+template <typename R, int FAST>
 int batch_blocks_per_cu(int limiter, bool nonideal, bool remap)
 {
     int n = 0;
     hipError_t e;
     if (limiter == kMinmod) {
-        e = nonideal ? detail::batch_occupancy<kMinmod, 1, R>(remap, &n)
-                     : detail::batch_occupancy<kMinmod, 0, R>(remap, &n);
+        e = nonideal ? detail::batch_occupancy<kMinmod, 1, R, FAST>(remap, &n)
+                     : detail::batch_occupancy<kMinmod, 0, R, FAST>(remap, &n);
     } else {
-        e = nonideal ? detail::batch_occupancy<kMC, 1, R>(remap, &n)
-                     : detail::batch_occupancy<kMC, 0, R>(remap, &n);
+        e = nonideal ? detail::batch_occupancy<kMC, 1, R, FAST>(remap, &n)
+                     : detail::batch_occupancy<kMC, 0, R, FAST>(remap, &n);
     }
     return (e == hipSuccess && n > 0) ? n : 1;
 }
@@ -213,18 +218,20 @@ hipError_t launch_rel_l1(const RA* A, const RB* B, const Geom& g, DD* partial, d
     return hipGetLastError();
 }
 
-template <typename R>
+template <typename R, int FAST>
 int stage_blocks_per_cu(int limiter, bool nonideal, bool remap)
 {
     int n = 0;
     hipError_t e;
     auto q = [&](auto kern) { return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, kBlock, 0); };
     if (limiter == kMinmod) {
-        e = nonideal ? (remap ? q(k_stage<2, kMinmod, 1, 1, R>) : q(k_stage<2, kMinmod, 1, 0, R>))
-                     : (remap ? q(k_stage<2, kMinmod, 0, 1, R>) : q(k_stage<2, kMinmod, 0, 0, R>));
+        e = nonideal ? (remap ? q(k_stage<2, kMinmod, 1, 1, R, FAST>)
+                              : q(k_stage<2, kMinmod, 1, 0, R, FAST>))
+                     : (remap ? q(k_stage<2, kMinmod, 0, 1, R, FAST>)
+                              : q(k_stage<2, kMinmod, 0, 0, R, FAST>));
     } else {
-        e = nonideal ? (remap ? q(k_stage<2, kMC, 1, 1, R>) : q(k_stage<2, kMC, 1, 0, R>))
-                     : (remap ? q(k_stage<2, kMC, 0, 1, R>) : q(k_stage<2, kMC, 0, 0, R>));
+        e = nonideal ? (remap ? q(k_stage<2, kMC, 1, 1, R, FAST>) : q(k_stage<2, kMC, 1, 0, R, FAST>))
+                     : (remap ? q(k_stage<2, kMC, 0, 1, R, FAST>) : q(k_stage<2, kMC, 0, 0, R, FAST>));
     }
     return (e == hipSuccess && n > 0) ? n : 1;
 }
@@ -246,5 +253,17 @@ int stage_blocks_per_cu(int limiter, bool nonideal, bool remap)
     template hipError_t launch_totals<R>(const R*, const Geom&, DD*, double*, hipStream_t);     \
     template hipError_t launch_max_div_b<R>(const R*, const Geom&, double, double,              \
                                             unsigned long long*, hipStream_t);
+
+// The stepping launchers of the fast fp64 build (mhd_fp64fast.hip).
+#define MHDHIP_INSTANTIATE_FAST(R)                                                               \
+    template hipError_t launch_cfl<R, 1>(const R*, const Geom&, const Physics&, const uint8_t*,  \
+                                         Ctrl*, hipStream_t);                                    \
+    template hipError_t launch_stage<R, 1>(int, int, bool, bool, const StageBuffers&,            \
+                                           const Geom&, const Physics&, const Tiling&, int,      \
+                                           hipStream_t);                                         \
+    template hipError_t launch_batch_stage<R, 1>(int, int, bool, bool, const BatchTables&, int,  \
+                                                 int, hipStream_t);                              \
+    template int batch_blocks_per_cu<R, 1>(int, bool, bool);                                     \
+    template int stage_blocks_per_cu<R, 1>(int, bool, bool);
 
 }  // namespace mhdhip

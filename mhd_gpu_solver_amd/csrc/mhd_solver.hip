@@ -208,8 +208,9 @@ struct mhd_hip_solver {
         if (g.nx < 1 || g.ny < 1) {
             throw InvalidArgument("grid must have at least one interior cell per axis");
         }
-        if (prec != MHD_HIP_FP64 && prec != MHD_HIP_FP32) {
-            throw InvalidArgument("precision must be MHD_HIP_FP64 or MHD_HIP_FP32");
+        if (prec != MHD_HIP_FP64 && prec != MHD_HIP_FP32 && prec != MHD_HIP_FP64_FAST) {
+            throw InvalidArgument(
+                "precision must be MHD_HIP_FP64, MHD_HIP_FP32 or MHD_HIP_FP64_FAST");
         }
         int count = 0;
         if (hipGetDeviceCount(&count) != hipSuccess || count < 1) {
@@ -221,7 +222,7 @@ struct mhd_hip_solver {
         }
         grid = g;
         precision = prec;
-        elem = prec == MHD_HIP_FP64 ? sizeof(double) : sizeof(float);
+        elem = prec == MHD_HIP_FP32 ? sizeof(float) : sizeof(double);
         geom.nx = g.nx;
         geom.ny = g.ny;
         geom.ng = g.ng;
@@ -312,14 +313,14 @@ struct mhd_hip_solver {
         }
         check(hipSetDevice(device), "hipSetDevice");
         if (9 * n * elem >= mhdhip::StagedCopier::kMinBytes) {
-            if (precision == MHD_HIP_FP64) {
+            if (precision != MHD_HIP_FP32) {
                 staging.upload(static_cast<double*>(U), plane(), comps, 9, n, stream);
             } else {
                 staging.upload(static_cast<float*>(U), plane(), comps, 9, n, stream);
             }
             return;
         }
-        if (precision == MHD_HIP_FP64) {
+        if (precision != MHD_HIP_FP32) {
             for (int c = 0; c < 9; ++c) {
                 copy_sync(static_cast<double*>(U) + c * plane(), comps[c], n * sizeof(double),
                           hipMemcpyHostToDevice, "hipMemcpy(upload)");
@@ -344,14 +345,14 @@ struct mhd_hip_solver {
         check(hipSetDevice(device), "hipSetDevice");
         check(hipStreamSynchronize(stream), "hipStreamSynchronize");
         if (9 * n * elem >= mhdhip::StagedCopier::kMinBytes) {
-            if (precision == MHD_HIP_FP64) {
+            if (precision != MHD_HIP_FP32) {
                 staging.download(comps, static_cast<const double*>(U), plane(), 9, n, stream);
             } else {
                 staging.download(comps, static_cast<const float*>(U), plane(), 9, n, stream);
             }
             return;
         }
-        if (precision == MHD_HIP_FP64) {
+        if (precision != MHD_HIP_FP32) {
             for (int c = 0; c < 9; ++c) {
                 copy_sync(comps[c], static_cast<double*>(U) + c * plane(), n * sizeof(double),
                           hipMemcpyDeviceToHost, "hipMemcpy(download)");
@@ -408,7 +409,7 @@ struct mhd_hip_solver {
                (p.ambipolar && p.eta_ambipolar != 0.0);
     }
 
-    template <typename R>
+    template <typename R, int FAST>
     void enqueue_step(const Physics& ph, const Tiling& tl, int limiter, bool ni, int parity)
     {
         const bool fill = masked && walls;
@@ -423,7 +424,7 @@ struct mhd_hip_solver {
             check(mhdhip::launch_wall_fill<R>(u, geom, plan_dev, plan_count, ph.gamma, gate, stream),
                   "wall_fill");
         }
-        check(mhdhip::launch_stage<R>(1, limiter, ni, remap, b1, geom, ph, tl, parity, stream),
+        check(mhdhip::launch_stage<R, FAST>(1, limiter, ni, remap, b1, geom, ph, tl, parity, stream),
               "stage1");
         if (fill) {
             check(mhdhip::launch_ghost_fill<R>(us, geom, gate, stream), "ghost_fill");
@@ -431,13 +432,13 @@ struct mhd_hip_solver {
                                               stream),
                   "wall_fill");
         }
-        check(mhdhip::launch_stage<R>(2, limiter, ni, remap, b2, geom, ph, tl, parity, stream),
+        check(mhdhip::launch_stage<R, FAST>(2, limiter, ni, remap, b2, geom, ph, tl, parity, stream),
               "stage2");
     }
 
     // Eager launches of `steps` steps with an event pair around every stage kernel; adds the
     // kernel times to stage_ms. Returns the number of steps launched (even).
-    template <typename R>
+    template <typename R, int FAST>
     long long timed_steps(const Physics& ph, const Tiling& tl, int limiter, bool ni, int steps)
     {
         const long long n = steps + (steps & 1);
@@ -464,7 +465,7 @@ struct mhd_hip_solver {
                       "wall_fill");
             }
             check(hipEventRecord(e[0], stream), "hipEventRecord");
-            check(mhdhip::launch_stage<R>(1, limiter, ni, remap, b1, geom, ph, tl, parity, stream),
+            check(mhdhip::launch_stage<R, FAST>(1, limiter, ni, remap, b1, geom, ph, tl, parity, stream),
                   "stage1");
             check(hipEventRecord(e[1], stream), "hipEventRecord");
             if (fill) {
@@ -474,7 +475,7 @@ struct mhd_hip_solver {
                       "wall_fill");
             }
             check(hipEventRecord(e[2], stream), "hipEventRecord");
-            check(mhdhip::launch_stage<R>(2, limiter, ni, remap, b2, geom, ph, tl, parity, stream),
+            check(mhdhip::launch_stage<R, FAST>(2, limiter, ni, remap, b2, geom, ph, tl, parity, stream),
                   "stage2");
             check(hipEventRecord(e[3], stream), "hipEventRecord");
         }
@@ -499,7 +500,7 @@ struct mhd_hip_solver {
     // (U* is scratch until the next stage 1; the control fields the two launches write are
     // rewritten by the step's own stages). Runs after the first CFL reduction of a solve, so
     // the kernels see a live step.
-    template <typename R>
+    template <typename R, int FAST>
     int tuned_rows(const Physics& ph, int limiter, bool ni)
     {
         const bool remap = !(masked && walls);
@@ -535,7 +536,7 @@ struct mhd_hip_solver {
         {
             hipDeviceProp_t prop;
             check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
-            const int per_cu = mhdhip::stage_blocks_per_cu<R>(limiter, ni, remap);
+            const int per_cu = mhdhip::stage_blocks_per_cu<R, FAST>(limiter, ni, remap);
             const long long slots =
                 static_cast<long long>(per_cu) * mhdhip::kWavesPerBlock * prop.multiProcessorCount;
             const long long strips = (grid.nx + mhdhip::kStripCells - 1) / mhdhip::kStripCells;
@@ -563,17 +564,17 @@ struct mhd_hip_solver {
                 }
                 const Tiling tl = mhdhip::make_tiling(grid.nx, grid.ny, cand);
                 // one untimed pair, then the best of three
-                check(mhdhip::launch_stage<R>(1, limiter, ni, remap, b1, geom, ph, tl, 0, stream),
+                check(mhdhip::launch_stage<R, FAST>(1, limiter, ni, remap, b1, geom, ph, tl, 0, stream),
                       "tune");
-                check(mhdhip::launch_stage<R>(2, limiter, ni, remap, b2, geom, ph, tl, 0, stream),
+                check(mhdhip::launch_stage<R, FAST>(2, limiter, ni, remap, b2, geom, ph, tl, 0, stream),
                       "tune");
                 float cand_ms = 1e30f;
                 for (int rep = 0; rep < 3; ++rep) {
                     check(hipEventRecord(e0, stream), "hipEventRecord");
-                    check(mhdhip::launch_stage<R>(1, limiter, ni, remap, b1, geom, ph, tl, 0,
+                    check(mhdhip::launch_stage<R, FAST>(1, limiter, ni, remap, b1, geom, ph, tl, 0,
                                                   stream),
                           "tune");
-                    check(mhdhip::launch_stage<R>(2, limiter, ni, remap, b2, geom, ph, tl, 0,
+                    check(mhdhip::launch_stage<R, FAST>(2, limiter, ni, remap, b2, geom, ph, tl, 0,
                                                   stream),
                           "tune");
                     check(hipEventRecord(e1, stream), "hipEventRecord");
@@ -608,7 +609,7 @@ struct mhd_hip_solver {
         graph_key.clear();
     }
 
-    template <typename R>
+    template <typename R, int FAST>
     void ensure_graph(const Physics& ph, const Tiling& tl, int limiter, bool ni)
     {
         std::vector<unsigned char> key(sizeof(Physics) + sizeof(Tiling) + 2 * sizeof(int) +
@@ -630,7 +631,7 @@ struct mhd_hip_solver {
         check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "BeginCapture");
         try {
             for (int s = 0; s < kSteps; ++s) {
-                enqueue_step<R>(ph, tl, limiter, ni, s & 1);
+                enqueue_step<R, FAST>(ph, tl, limiter, ni, s & 1);
             }
         } catch (...) {
             (void)hipStreamEndCapture(stream, &graph);
@@ -644,7 +645,7 @@ struct mhd_hip_solver {
         graph_key = key;
     }
 
-    template <typename R>
+    template <typename R, int FAST>
     mhd_hip_result run(const mhd_hip_params& p, double t_end, int max_steps, bool allow_max_steps)
     {
         if (t_end < 0.0) {
@@ -666,17 +667,17 @@ struct mhd_hip_solver {
 
         check(hipEventRecord(ev_start, stream), "hipEventRecord");
         check(mhdhip::launch_ctrl_init<R>(ctrl, stream), "ctrl_init");
-        check(mhdhip::launch_cfl<R>(u, geom, ph, masked ? mask_dev : nullptr, ctrl, stream), "cfl");
-        const int rows = seg_rows > 0 ? seg_rows : tuned_rows<R>(ph, p.limiter, ni);
+        check(mhdhip::launch_cfl<R, FAST>(u, geom, ph, masked ? mask_dev : nullptr, ctrl, stream), "cfl");
+        const int rows = seg_rows > 0 ? seg_rows : tuned_rows<R, FAST>(ph, p.limiter, ni);
         const Tiling tl = mhdhip::make_tiling(grid.nx, grid.ny, rows);
         last_rows = rows;
 
         long long launched = 0;
         const bool any_step = t_end > 0.0 && max_steps > 0;
         if (any_step && kernel_timing && allow_max_steps) {
-            launched = timed_steps<R>(ph, tl, p.limiter, ni, max_steps);
+            launched = timed_steps<R, FAST>(ph, tl, p.limiter, ni, max_steps);
         } else if (any_step) {
-            ensure_graph<R>(ph, tl, p.limiter, ni);
+            ensure_graph<R, FAST>(ph, tl, p.limiter, ni);
             // Two replays in flight: queue replay k+1, then inspect the control block after k.
             hipEvent_t done[2];
             check(hipEventCreateWithFlags(&done[0], hipEventDisableTiming), "hipEventCreate");
@@ -767,8 +768,11 @@ struct mhd_hip_solver {
 
     mhd_hip_result solve(const mhd_hip_params& p)
     {
-        return precision == MHD_HIP_FP64 ? run<double>(p, p.t_end, p.max_steps, false)
-                                         : run<float>(p, p.t_end, p.max_steps, false);
+        switch (precision) {
+            case MHD_HIP_FP64: return run<double, 0>(p, p.t_end, p.max_steps, false);
+            case MHD_HIP_FP64_FAST: return run<double, 1>(p, p.t_end, p.max_steps, false);
+            default: return run<float, 0>(p, p.t_end, p.max_steps, false);
+        }
     }
 
     mhd_hip_result advance(const mhd_hip_params& p, int steps)
@@ -777,8 +781,11 @@ struct mhd_hip_solver {
             throw InvalidArgument("steps must be non-negative");
         }
         const double inf = std::numeric_limits<double>::infinity();
-        return precision == MHD_HIP_FP64 ? run<double>(p, inf, steps, true)
-                                         : run<float>(p, inf, steps, true);
+        switch (precision) {
+            case MHD_HIP_FP64: return run<double, 0>(p, inf, steps, true);
+            case MHD_HIP_FP64_FAST: return run<double, 1>(p, inf, steps, true);
+            default: return run<float, 0>(p, inf, steps, true);
+        }
     }
 
     template <typename R>
@@ -798,7 +805,7 @@ struct mhd_hip_solver {
     void fill_boundaries(double gamma, int what)
     {
         check(hipSetDevice(device), "hipSetDevice");
-        if (precision == MHD_HIP_FP64) {
+        if (precision != MHD_HIP_FP32) {
             fill_boundaries_t<double>(gamma, what);
         } else {
             fill_boundaries_t<float>(gamma, what);
@@ -816,7 +823,7 @@ struct mhd_hip_solver {
         check(hipSetDevice(device), "hipSetDevice");
         auto* partial = reinterpret_cast<mhdhip::DD*>(base);
         auto* dev_out = reinterpret_cast<double*>(base + kNoallocPartials * sizeof(mhdhip::DD));
-        if (precision == MHD_HIP_FP64) {
+        if (precision != MHD_HIP_FP32) {
             check(mhdhip::launch_totals<double>(static_cast<double*>(U), geom, partial, dev_out,
                                                 stream),
                   "totals");
@@ -840,7 +847,7 @@ struct mhd_hip_solver {
             check(hipMalloc(reinterpret_cast<void**>(&totals_out), 9 * sizeof(double)),
                   "hipMalloc(totals)");
         }
-        if (precision == MHD_HIP_FP64) {
+        if (precision != MHD_HIP_FP32) {
             check(mhdhip::launch_totals<double>(static_cast<double*>(U), geom, totals_partial,
                                                 totals_out, stream),
                   "totals");
@@ -869,8 +876,8 @@ struct mhd_hip_solver {
                   "hipMalloc(totals)");
         }
         check(hipStreamSynchronize(ref.stream), "hipStreamSynchronize");
-        check(mhdhip::launch_rel_l1_any(U, precision == MHD_HIP_FP64, ref.U,
-                                        ref.precision == MHD_HIP_FP64, geom, l1_partial,
+        check(mhdhip::launch_rel_l1_any(U, precision != MHD_HIP_FP32, ref.U,
+                                        ref.precision != MHD_HIP_FP32, geom, l1_partial,
                                         totals_out, stream),
               "rel_l1");
         check(hipMemcpyAsync(out, totals_out, 9 * sizeof(double), hipMemcpyDeviceToHost, stream),
@@ -889,7 +896,7 @@ struct mhd_hip_solver {
         const double dx = (grid.x_max - grid.x_min) / static_cast<double>(grid.nx);
         const double dy = (grid.y_max - grid.y_min) / static_cast<double>(grid.ny);
         auto* out = reinterpret_cast<unsigned long long*>(totals_out);
-        if (precision == MHD_HIP_FP64) {
+        if (precision != MHD_HIP_FP32) {
             check(mhdhip::launch_max_div_b<double>(static_cast<double*>(U), geom, 0.5 / dx,
                                                    0.5 / dy, out, stream),
                   "max_div_b");

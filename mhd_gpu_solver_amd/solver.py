@@ -165,7 +165,8 @@ class DeviceSolver:
         return buf.value.decode()
 
     def precision(self) -> str:
-        return "fp64" if self._l.mhd_hip_precision(self.handle) == _lib.FP64 else "fp32"
+        code = self._l.mhd_hip_precision(self.handle)
+        return next(k for k, v in _lib.PRECISIONS.items() if v == code)
 
     def configure(self, bc_x, bc_y, mask, reflective_walls):
         if mask is None:
@@ -468,7 +469,9 @@ class MHDSolver:
                                "(this package provides the hip backend)")
         else:
             raise ValueError(f"Unknown backend: {backend} (expected hip)")
-        self._precision = "fp32" if precision == "fp32" else "fp64"
+        # solver_api.hpp:180-184: "fp32" selects FP32 and any other string fp64; "fp64_fast"
+        # (this backend's contracted fp64 build) is the one name added
+        self._precision = precision if precision in ("fp32", "fp64_fast") else "fp64"
         self._device = None
 
     @property
@@ -484,8 +487,7 @@ class MHDSolver:
 
     def _dev(self) -> DeviceSolver:
         if self._device is None:
-            self._device = DeviceSolver(self._grid,
-                                        _lib.FP32 if self._precision == "fp32" else _lib.FP64)
+            self._device = DeviceSolver(self._grid, _lib.PRECISIONS[self._precision])
         return self._device
 
     # ---- time integration (solver_api.hpp:294-329) -----------------------------------------------

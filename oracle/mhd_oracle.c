@@ -787,9 +787,12 @@ double orc_cfl(const orc_grid* g, const double* U, const orc_params* p, const ui
 
 /* ---- SSP-RK2 and the solve loop: include/mhd/time_integrator.hpp:36-106,
  *      include/mhd/field_ops.hpp:47-63 (x + a*y, 0.5*(u+s) + (0.5*dt)*r) ---------------- */
-int orc_solve(const orc_grid* g, int bc_x, int bc_y, const uint8_t* mask, int reflective_walls,
-              const orc_params* p, int allow_max_steps, double* U, double* dt_log, int dt_cap,
-              double* t_out, int* steps_out, double* ch_out)
+/* The solve loop; with `forced` it takes exactly `nforced` steps whose dt are forced[k]
+ * instead of the CFL dt (c_h of each step still comes from the CFL reduction). */
+static int solve_impl(const orc_grid* g, int bc_x, int bc_y, const uint8_t* mask,
+                      int reflective_walls, const orc_params* p, int allow_max_steps, double* U,
+                      double* dt_log, int dt_cap, const double* forced, int nforced,
+                      double* t_out, int* steps_out, double* ch_out)
 {
     if (p->t_end < 0.0 || p->cfl <= 0.0 || g->ng < 2 || g->nx < 1 || g->ny < 1) {
         return ORC_EINVAL;
@@ -814,12 +817,12 @@ int orc_solve(const orc_grid* g, int bc_x, int bc_y, const uint8_t* mask, int re
     double t = 0.0;
     int steps = 0;
     double c_h_last = 0.0;
-    while (t < p->t_end && steps < p->max_steps) {
+    while (forced != NULL ? steps < nforced : (t < p->t_end && steps < p->max_steps)) {
         double c_h;
         const double dt_cfl = orc_cfl(g, U, p, m, &c_h);
         c_h_last = c_h;
         const double rem = p->t_end - t;
-        const double dt = vmin(dt_cfl, rem);
+        const double dt = forced != NULL ? forced[steps] : vmin(dt_cfl, rem);
         if (dt_log != NULL && steps < dt_cap) {
             dt_log[steps] = dt;
         }
@@ -848,7 +851,7 @@ int orc_solve(const orc_grid* g, int bc_x, int bc_y, const uint8_t* mask, int re
         ++steps;
     }
     int rc = ORC_OK;
-    if (t < p->t_end && steps >= p->max_steps && !allow_max_steps) {
+    if (forced == NULL && t < p->t_end && steps >= p->max_steps && !allow_max_steps) {
         rc = ORC_EMAXSTEPS;
     }
     orc_apply_bc(g, U, bc_x, bc_y);
@@ -868,6 +871,28 @@ int orc_solve(const orc_grid* g, int bc_x, int bc_y, const uint8_t* mask, int re
         *ch_out = c_h_last;
     }
     return rc;
+}
+
+/* include/mhd/time_integrator.hpp:62-106 (solve) */
+int orc_solve(const orc_grid* g, int bc_x, int bc_y, const uint8_t* mask, int reflective_walls,
+              const orc_params* p, int allow_max_steps, double* U, double* dt_log, int dt_cap,
+              double* t_out, int* steps_out, double* ch_out)
+{
+    return solve_impl(g, bc_x, bc_y, mask, reflective_walls, p, allow_max_steps, U, dt_log,
+                      dt_cap, NULL, 0, t_out, steps_out, ch_out);
+}
+
+/* The same steps with a given dt sequence (time_integrator.hpp:36-60 per step): compares a
+ * solver whose dt may differ in the last bits (a contracted fp64 build) at its own dt. */
+int orc_solve_dt(const orc_grid* g, int bc_x, int bc_y, const uint8_t* mask, int reflective_walls,
+                 const orc_params* p, double* U, const double* dts, int nsteps, double* t_out,
+                 int* steps_out, double* ch_out)
+{
+    if (dts == NULL || nsteps < 0) {
+        return ORC_EINVAL;
+    }
+    return solve_impl(g, bc_x, bc_y, mask, reflective_walls, p, 1, U, NULL, 0, dts, nsteps,
+                      t_out, steps_out, ch_out);
 }
 
 /* ---- reference-named initial conditions: include/initial_condition/inititial_condition.hpp

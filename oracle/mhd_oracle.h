@@ -85,6 +85,11 @@ int orc_solve(const orc_grid* g, int bc_x, int bc_y, const uint8_t* mask, int re
               const orc_params* p, int allow_max_steps, double* U, double* dt_log, int dt_cap,
               double* t_out, int* steps_out, double* ch_out);
 
+/* orc_solve with exactly `nsteps` steps of the given dt sequence (c_h from the CFL reduction). */
+int orc_solve_dt(const orc_grid* g, int bc_x, int bc_y, const uint8_t* mask, int reflective_walls,
+                 const orc_params* p, double* U, const double* dts, int nsteps, double* t_out,
+                 int* steps_out, double* ch_out);
+
 /* max |dBx/dx + dBy/dy| over interior cells by central differences of U as stored (ghosts
  * included), max by std::max rule. Restates inititial_condition.hpp:273-287. */
 double orc_max_div_b(const orc_grid* g, const double* U);

@@ -98,6 +98,11 @@ class Oracle:
                                   C.POINTER(Params), C.c_int, _dp, C.c_void_p, C.c_int,
                                   C.POINTER(C.c_double), C.POINTER(C.c_int),
                                   C.POINTER(C.c_double)]
+        lib.orc_solve_dt.restype = C.c_int
+        lib.orc_solve_dt.argtypes = [C.POINTER(Grid), C.c_int, C.c_int, _u8, C.c_int,
+                                     C.POINTER(Params), _dp, _dp, C.c_int,
+                                     C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_double)]
         lib.orc_init_ic.restype = C.c_int
         lib.orc_init_ic.argtypes = [C.c_char_p, C.POINTER(Grid), C.c_double, _dp]
         lib.orc_apply_bc.restype = C.c_int
@@ -143,6 +148,17 @@ class Oracle:
             raise RuntimeError("oracle: reached max_steps before t_end")
         return U, dict(t=t.value, steps=steps.value, c_h=ch.value,
                        dt=log[:min(steps.value, dt_cap)] if dt_cap else None)
+
+    def solve_dt(self, g, U, params, dts, bc=(PERIODIC, PERIODIC), mask=None, walls=True):
+        """len(dts) steps with the given dt sequence (c_h from each step's CFL reduction)."""
+        U = np.ascontiguousarray(U, dtype=np.float64).copy()
+        d = np.ascontiguousarray(dts, dtype=np.float64)
+        t, steps, ch = C.c_double(), C.c_int(), C.c_double()
+        m, mp = _mask_ptr(mask)
+        if self.lib.orc_solve_dt(C.byref(g), bc[0], bc[1], mp, int(walls), C.byref(params), U, d,
+                                 len(d), C.byref(t), C.byref(steps), C.byref(ch)) != 0:
+            raise ValueError("oracle: invalid argument")
+        return U, dict(t=t.value, steps=steps.value, c_h=ch.value)
 
     def init_ic(self, name, g, gamma=5.0 / 3.0, bc=(PERIODIC, PERIODIC)):
         U = empty_state(g)

@@ -351,3 +351,18 @@ def test_degenerate_span_average_matches_closed_form(oracle):
         e = p / (5.0 / 3.0 - 1.0) + 0.5 * rho * vn * vn
         return np.array([rho * vn, rho * vn * vn + p, 0, 0, (e + p) * vn, 0, 0, 0, 0])
     np.testing.assert_allclose(F, 0.5 * (flux(WL) + flux(WR)), rtol=1e-14, atol=1e-60)
+
+
+def test_forced_dt_solve_reproduces_the_cfl_solve(oracle):
+    """orc_solve_dt driven by a solve's own dt sequence reproduces that solve bit for bit (the
+    mode the contracted fp64 build is compared in), and a different dt gives another state."""
+    g = make_grid(40, 36)
+    U0 = oracle.init_ic("orszag_tang", g)
+    p = make_params(t_end=0.03, limiter=MC, eta_ohm=1e-3, eta_hall=1e-3)
+    U1, info = oracle.solve(g, U0, p, dt_cap=64)
+    U2, info2 = oracle.solve_dt(g, U0, p, info["dt"])
+    assert info2["steps"] == info["steps"] and info2["t"] == info["t"]
+    assert info2["c_h"] == info["c_h"]
+    assert np.array_equal(U1, U2)
+    U3, _ = oracle.solve_dt(g, U0, p, info["dt"] * (1.0 + 1e-9))
+    assert not np.array_equal(U1, U3)
