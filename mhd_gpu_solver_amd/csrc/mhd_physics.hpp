@@ -103,10 +103,26 @@ __device__ __forceinline__ R minmod(R a, R b)
     return (a * b <= zero) ? zero : ((dabs(a) < dabs(b)) ? a : b);
 }
 
+// fp32: minmod(a, b) is the median of a, b and 0 (same signs: the one of smaller magnitude;
+// opposite signs or a zero: 0), one v_med3_f32. It differs from the select form only where
+// a * b underflows to 0 with both of one sign (the select form gives 0) and in the NaN it
+// propagates; the FP32 path is held to a relative L1 tolerance, the fp64 path keeps the form
+// above.
+__device__ __forceinline__ float minmod_med3(float a, float b)
+{
+    return __builtin_amdgcn_fmed3f(a, b, 0.0f);
+}
+
 template <int LIM, typename R>
 __device__ __forceinline__ R limit_slope(R a, R b)
 {
-    if constexpr (LIM == kMinmod) {
+    if constexpr (sizeof(R) == 4) {
+        if constexpr (LIM == kMinmod) {
+            return minmod_med3(a, b);
+        } else {
+            return minmod_med3(0.5f * (a + b), minmod_med3(2.0f * a, 2.0f * b));
+        }
+    } else if constexpr (LIM == kMinmod) {
         return minmod(a, b);
     } else {
         const R half(0.5);
