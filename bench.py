@@ -100,16 +100,18 @@ def single_leg(m, n, precision, steps, warmup, U0):
                 cells=cells, drift=drift, pcie=pcie)
 
 
-def measured_traffic(kernel_prefix, elem_bytes, workload, kernel="k_stage"):
+def measured_traffic(kernel_prefix, elem_bytes, workload, kernel="k_stage", fast=0):
     """HBM bytes per launch of a stage kernel from the committed FETCH_SIZE/WRITE_SIZE summary
-    (profiles/traffic_<kernel>_<workload>.json), or None when none exists for this workload."""
+    (profiles/traffic_<kernel>_<workload>.json), or None when none exists for this workload.
+    The kernel is the unmasked ideal one of the precision (and, for fp64, of the exact or the
+    fast build: the last template argument)."""
     path = ROOT / "profiles" / f"traffic_{kernel}_{workload}.json"
     if not path.exists():
         return None
     data = json.loads(path.read_text())
-    want = "double" if elem_bytes == 8 else "float"
+    want = ("double" if elem_bytes == 8 else "float") + f", {fast}>"
     for name, v in data.get("kernels", {}).items():
-        if name.startswith(kernel_prefix) and want in name:
+        if name.startswith(kernel_prefix) and name.endswith(want):
             return v["traffic_bytes"]
     return None
 
@@ -413,8 +415,12 @@ def main():
                                   "ms_per_step": round(b64f["ms_per_step"], 4),
                                   "seg_rows": b64f["seg_rows"],
                                   "rel_l1_max_vs_fp64": float(f"{fast_l1:.3e}"),
-                                  "roofline": roofline(b64f["kernels"], b64f["cells"],
-                                                       b64f["ms_per_step"], 8, "k_batch_stage")},
+                                  "roofline": roofline(
+                                      b64f["kernels"], b64f["cells"], b64f["ms_per_step"], 8,
+                                      "k_batch_stage",
+                                      traffic=lambda dom: measured_traffic(
+                                          f"k_batch_stage<{dom + 1}", 8,
+                                          f"ensemble_{len(specs)}", "k_batch_stage", fast=1))},
             "batched_fp64": {"value": round(b64["mcups"], 2),
                              "drift_periodic_max": b64["drift"],
                              "ms_per_step": round(b64["ms_per_step"], 4),
@@ -456,9 +462,9 @@ def main():
                 "drift": leg["drift"], "pcie_inclusive": leg["pcie"],
                 "roofline": roofline(leg["kernels"], leg["cells"], leg["ms_per_step"], eb,
                                      "k_stage",
-                                     traffic=None if prec == "fp64_fast" else
-                                     (lambda dom, eb=eb: measured_traffic(
-                                         f"k_stage<{dom + 1}", eb, f"orszag_tang_{n}"))),
+                                     traffic=lambda dom, eb=eb, prec=prec: measured_traffic(
+                                         f"k_stage<{dom + 1}", eb, f"orszag_tang_{n}",
+                                         fast=int(prec == "fp64_fast"))),
             }
         out["single_run"] = {"workload": f"orszag_tang_{n}x{n}_single_run", **legs}
         if args.only == "single":
