@@ -200,10 +200,13 @@ struct mhd_hip_batch {
     int sched_policy = MHD_HIP_SCHED_ALL;
     long long sched_cap = 0;  // interior cells stepping per replay under SRPT
     // Work partition: > 0 fixed segments of that many rows, handed out round-robin; 0 the load
-    // model's length at every table build; -1 timed once over the candidates; < -1 the
-    // capacity fill with pieces of at least -seg_rows rows.
+    // model's length at every table build; -1 timed on real steps over a shortlist, then kept;
+    // < -1 the capacity fill with pieces of at least -seg_rows rows.
     int seg_rows = 0;
     int tuned_rows = 0;  // the timed choice: > 0 fixed rows, < 0 fill with -tuned_rows min rows
+    std::vector<int> tune_modes;  // the partitions being timed (empty: none yet, or decided)
+    std::vector<float> tune_ms;   // fastest timed step of each (ms)
+    std::vector<int> tune_count;  // timed steps of each
     int last_rows = 0;  // fixed rows of the current tile list (0: capacity fill)
     int ntiles = 0;
     int blocks = 0;                      // the largest persistent grid of the variants
@@ -650,11 +653,8 @@ struct mhd_hip_batch {
         dirty = true;
     }
 
-    // Row lengths the load model weighs, and those the timed choice (rows per wave -1) tries
-    // beside the capacity fill with minimum pieces of kFillMinRows rows.
+    // Row lengths the load model weighs (and the timed choice shortlists from).
     static constexpr int kModelRowCandidates[] = {8, 12, 16, 20, 24, 28, 32, 40, 48, 64, 96, 128};
-    static constexpr int kTimedRowCandidates[] = {8, 16, 24, 32, 48, 64};
-    static constexpr int kFillMinRows[] = {32};
 
     int group_waves(int g) const { return group_blocks[g] * mhdhip::kWavesPerBlock; }
 
@@ -816,36 +816,93 @@ struct mhd_hip_batch {
         graph_key = launch_key();
     }
 
-    // Timed choice (rows per wave -1): times one stage-1 pass (which leaves U's interior
-    // untouched) over the current instance set for the capacity fill and each fixed row
-    // length, and keeps the fastest partition.
-    template <typename R, int FAST>
-    void autotune()
+    // Timed choice (rows per wave -1). The first replays are tuning replays: their steps are
+    // real steps of every stepping instance (results never depend on the partition), taken in
+    // pairs with one shortlisted partition each, the one timed least often so far; the second
+    // step of a pair, which follows a step with the same partition as in a run, is timed with
+    // events around its two stage launches. Once every candidate has a timing the fastest is
+    // kept for the engine's life. Shortlist: up to three fixed row lengths, those of least
+    // load-model cost (summed over the kernel variants in use) within 12 % of the least.
+    std::vector<int> shortlist() const
     {
-        std::vector<int> modes;
-        for (const int m : kFillMinRows) modes.push_back(-m);
-        for (const int rows : kTimedRowCandidates) modes.push_back(rows);
-        int best = modes.front();
-        float best_ms = 1e30f;
-        for (const int rows : modes) {
-            build_tables(rows);
-            launch_stage<R, FAST>(1, 0);
-            float ms_best = 1e30f;
-            for (int rep = 0; rep < 2; ++rep) {
-                check(hipEventRecord(ev0, stream), "hipEventRecord");
-                launch_stage<R, FAST>(1, 0);
-                check(hipEventRecord(ev1, stream), "hipEventRecord");
-                check(hipEventSynchronize(ev1), "hipEventSynchronize");
-                float ms = 0.0f;
-                check(hipEventElapsedTime(&ms, ev0, ev1), "hipEventElapsedTime");
-                ms_best = std::min(ms_best, ms);
-            }
-            if (ms_best < best_ms) {
-                best_ms = ms_best;
-                best = rows;
-            }
+        std::vector<std::pair<long long, int>> cost;
+        for (const int rows : kModelRowCandidates) cost.emplace_back(0, rows);
+        for (int g = 0; g < kGroups; ++g) {
+            const std::vector<mhdhip::PlanInst> insts = stepping_set(g);
+            if (insts.empty()) continue;
+            for (auto& c : cost) c.first += mhdhip::fixed_max_cost(insts, c.second, group_waves(g));
         }
-        tuned_rows = best;
+        std::sort(cost.begin(), cost.end());
+        std::vector<int> modes;
+        for (size_t i = 0; i < cost.size() && i < 3; ++i) {
+            if (cost[i].first * 100 <= cost.front().first * 112) modes.push_back(cost[i].second);
+        }
+        return modes;
+    }
+
+    template <typename R, int FAST>
+    void tuning_replay()
+    {
+        if (tune_modes.empty()) {
+            tune_modes = shortlist();
+            tune_ms.assign(tune_modes.size(), 1e30f);
+            tune_count.assign(tune_modes.size(), 0);
+        }
+        std::vector<int> before(static_cast<size_t>(max_instances), 0);
+        std::vector<char> stepping(static_cast<size_t>(max_instances), 0);
+        for (int si = 0; si < max_instances; ++si) {
+            const Slot& s = slots[si];
+            stepping[si] = s.used && !s.finished && !s.paused;
+            before[si] = h_ctrl[si].steps[0];
+        }
+        while (tev.size() < 4 * kSteps) {
+            hipEvent_t e;
+            check(hipEventCreate(&e), "hipEventCreate");
+            tev.push_back(e);
+        }
+        constexpr int kPairs = kSteps / 2;
+        std::vector<int> planned(tune_count), pick(kPairs, 0);
+        for (int j = 0; j < kPairs; ++j) {
+            pick[j] = static_cast<int>(std::min_element(planned.begin(), planned.end()) -
+                                       planned.begin());
+            ++planned[pick[j]];
+        }
+        for (int j = 0; j < kPairs; ++j) {
+            build_tables(tune_modes[pick[j]]);  // waits for the previous pair's launches
+            launch_stage<R, FAST>(1, 0);
+            launch_stage<R, FAST>(2, 0);
+            check(hipEventRecord(tev[2 * j], stream), "hipEventRecord");
+            launch_stage<R, FAST>(1, 1);
+            launch_stage<R, FAST>(2, 1);
+            check(hipEventRecord(tev[2 * j + 1], stream), "hipEventRecord");
+        }
+        check(hipMemcpyAsync(h_ctrl, d_ctrl, sizeof(Ctrl) * max_instances, hipMemcpyDeviceToHost,
+                             stream),
+              "hipMemcpyAsync(ctrl)");
+        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        std::vector<int> after(static_cast<size_t>(max_instances), 0);
+        for (int si = 0; si < max_instances; ++si) {
+            after[si] = h_ctrl[si].steps[0];
+        }
+        const int live = mhdhip::replay_active_launches(before, after, stepping, kSteps);
+        for (int j = 0; 2 * j + 1 < live; ++j) {
+            float ms = 0.0f;
+            check(hipEventElapsedTime(&ms, tev[2 * j], tev[2 * j + 1]), "hipEventElapsedTime");
+            tune_ms[pick[j]] = std::min(tune_ms[pick[j]], ms);
+            ++tune_count[pick[j]];
+        }
+        if (*std::min_element(tune_count.begin(), tune_count.end()) >= 1) {
+            tuned_rows = tune_modes[std::min_element(tune_ms.begin(), tune_ms.end()) -
+                                    tune_ms.begin()];
+            tune_modes.clear();
+            // the tables and the captured graph of the choice now, so that the first replay
+            // with it does not pay for them
+            build_tables(plan_mode());
+            dirty = false;
+            ensure_graph<R, FAST>();
+            check(hipGraphUpload(graph, stream), "hipGraphUpload");
+            return;
+        }
         dirty = true;
     }
 
@@ -984,14 +1041,15 @@ struct mhd_hip_batch {
                 last.valid = false;
                 break;
             }
+            note_launch(last);
             if (seg_rows == -1 && tuned_rows == 0) {
-                autotune<R, FAST>();
+                tuning_replay<R, FAST>();
+                continue;
             }
             if (dirty) {
                 build_tables(plan_mode());
                 dirty = false;
             }
-            note_launch(last);
             if (kernel_timing) {
                 timed_replay<R, FAST>();
             } else {
@@ -1276,6 +1334,7 @@ int mhd_hip_batch_set_seg_rows(mhd_hip_batch* b, int rows)
         if (!b) throw InvalidArgument("null batch");
         b->seg_rows = rows;
         b->tuned_rows = 0;
+        b->tune_modes.clear();
         b->dirty = true;
     });
 }

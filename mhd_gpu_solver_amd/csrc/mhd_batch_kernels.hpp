@@ -57,9 +57,13 @@ void k_batch_stage(BatchTables tb, int parity)
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = wave_in_block();
     const int gw = xcd_logical_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
-    const int t_end = tb.first[gw + 1];
+    const int t_begin = tb.first[gw];
+    const int n_tiles = tb.first[gw + 1] - t_begin;
     __shared__ R ring[kWavesPerBlock][3][kNC][kWave];
-    for (int t = tb.first[gw]; t < t_end; ++t) {
+    // stage 2 takes the wave's tiles in reverse order (and marches each upward), so that it
+    // starts on what stage 1 touched last, and the next stage 1 on what stage 2 touched last
+    for (int j = 0; j < n_tiles; ++j) {
+        const int t = STAGE == 2 ? t_begin + n_tiles - 1 - j : t_begin + j;
         const BatchTile tile = tb.tiles[t];
         const BatchInst& in = tb.insts[tile.inst];
         if (!in.enabled) {
@@ -88,7 +92,7 @@ void k_batch_stage(BatchTables tb, int parity)
         const R* u0 = static_cast<const R*>(in.U);
         R* dst = static_cast<R*>(STAGE == 1 ? in.Us : in.U);
         const int dB = tile.r0b < 0 ? 0 : tile.r0b - tile.r0;
-        const R local = march_tile<STAGE, LIM, NI, REMAP, 2, 1, R>(
+        const R local = march_tile<STAGE, LIM, NI, REMAP, 2, 1, kMarchDir<STAGE>, R>(
             src, u0, dst, REMAP ? nullptr : in.mask, in.g, in.ph, st, tile.strip, tile.r0, tile.r1, dB,
             tile.r0b < 0 ? tile.r1 : tile.r1b, ring[wave], lane);
         if (STAGE == 2) {

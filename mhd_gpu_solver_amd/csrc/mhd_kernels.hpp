@@ -604,6 +604,11 @@ struct StageBuffers {
     int dt_cap;
 };
 
+// Stage 1 marches its rows in increasing order and stage 2 in decreasing order, so that each
+// stage starts on the rows the previous one touched last (still in the memory-side cache).
+template <int STAGE>
+constexpr int kMarchDir = STAGE == 2 ? -1 : 1;
+
 // March rows [r0, r1) of one 60-column strip (see the comment above StageBuffers) and apply the
 // RK stage to its cells. `ring` is this wave's LDS slice. Returns the wave-local maximum CFL
 // signal speed of the updated fluid cells (stage 2; 0 in stage 1).
@@ -617,7 +622,9 @@ struct StageBuffers {
 //
 // EARLY: stage 2 issues its loads of U for the updated row at the start of the row step
 // (latency overlapped with the faces, nine more live registers); otherwise at the update.
-template <int STAGE, int LIM, int NI, int REMAP, int HALF, int EARLY, typename R>
+//
+// DIR: the order in which the rows are visited (see the loop below).
+template <int STAGE, int LIM, int NI, int REMAP, int HALF, int EARLY, int DIR, typename R>
 __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, R* dst,
                                         const uint8_t* mask, const Geom& g, const Physics& ph,
                                         const StepState& st, int strip, int r0, int r1, int dB,
@@ -673,8 +680,15 @@ __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, 
         }
     };
 
+    // The march visits rows c = c_first, c_first + DIR, ..., c_last. Wa, Wb, Wn
+    // hold the primitives of rows c - DIR, c, c + DIR: with DIR = 1 (downward in memory order)
+    // from c = r0 - 1 to r1, the face c - 1/2 is formed at row c and row c - 1 updated; with
+    // DIR = -1 from c = r1 to r0 - 1, the face c + 1/2 and row c + 1. Both form every face from
+    // the same operands, so the results are the same bits.
+    const int c_first = DIR > 0 ? r0 - 1 : r1;
+    const int c_last = DIR > 0 ? r1 : r0 - 1;
     R Wa[kNC], Wb[kNC], Wn[kNC], Uraw[kNC];
-    R WLy[kNC], Gp[kNC];
+    R Wc[kNC], Gp[kNC];  // the carried face state of row c - DIR, the flux of its other face
     R Jp[3] = {R(0), R(0), R(0)};
     R local_speed = R(0);
 
@@ -687,37 +701,48 @@ __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, 
         }
     };
 
-    load_row(r0 - 2, Uraw);
+    load_row(c_first - DIR, Uraw);
     to_prim(Uraw, gm1, Wa);
-    load_row(r0 - 1, Uraw);
+    load_row(c_first, Uraw);
     to_prim(Uraw, gm1, Wb);
-    load_row(r0, Uraw);
+    load_row(c_first + DIR, Uraw);
 
-    for (int c = r0 - 1; c <= r1; ++c) {
-        ring_put(c + 1, Uraw);
-        to_prim(Uraw, gm1, Wn);  // row c + 1
-        if (c + 2 <= r1 + 1) {
-            load_row(c + 2, Uraw);
+    // (a row step's place in the march, from c alone: the first step forms no face, the
+    // second no update, the last loads no row)
+    for (int c = c_first; DIR > 0 ? c <= c_last : c >= c_last; c += DIR) {
+        const bool face = DIR > 0 ? c >= r0 : c <= r1 - 1;
+        const bool update = DIR > 0 ? c >= r0 + 1 : c <= r1 - 2;
+        ring_put(c + DIR, Uraw);
+        to_prim(Uraw, gm1, Wn);  // row c + DIR
+        if (DIR > 0 ? c + 2 <= r1 + 1 : c - 2 >= r0 - 2) {
+            load_row(c + 2 * DIR, Uraw);
         }
-        // stage 2 updates row c - 1 from U at the start of the step: with EARLY those loads are
-        // issued now, so that their latency overlaps the face computations (column clamped like
-        // the row loads)
+        // the rows below and above row c
+        const R (&Wlo)[kNC] = DIR > 0 ? Wa : Wn;
+        const R (&Whi)[kNC] = DIR > 0 ? Wn : Wa;
+        // stage 2 updates row c - DIR from U at the start of the step: with EARLY those loads
+        // are issued now, so that their latency overlaps the face computations (column clamped
+        // like the row loads)
         constexpr bool early_u0 = STAGE == 2 && EARLY;
         R U0r[kNC];
-        if (early_u0 && c >= r0 + 1) {
-            const long long base = static_cast<long long>(c - 1) * g.nxt + (gic + lane_off);
+        if (early_u0 && update) {
+            const long long base =
+                static_cast<long long>(c - DIR) * g.nxt + (gic + lane_off);
 #pragma unroll
             for (int k = 0; k < kNC; ++k) {
                 U0r[k] = u0[k * P + base];
             }
         }
-        // y slope of row c and the two face states it produces
-        R WRy[kNC], WLn[kNC];
+        // y slope of row c and the two face states it produces: toward row c - DIR (used now)
+        // and toward row c + DIR (carried)
+        R Wf[kNC], Wn_c[kNC];
 #pragma unroll
         for (int k = 0; k < kNC; ++k) {
-            const R d = limit_slope<LIM>(Wb[k] - Wa[k], Wn[k] - Wb[k]);
-            WRy[k] = Wb[k] - half * d;
-            WLn[k] = Wb[k] + half * d;
+            const R d = limit_slope<LIM>(Wb[k] - Wlo[k], Whi[k] - Wb[k]);
+            const R lo = Wb[k] - half * d;  // the state at face c - 1/2
+            const R hi = Wb[k] + half * d;  // the state at face c + 1/2
+            Wf[k] = DIR > 0 ? lo : hi;
+            Wn_c[k] = DIR > 0 ? hi : lo;
         }
         R Jc[3] = {R(0), R(0), R(0)};
         if (NI) {
@@ -726,18 +751,22 @@ __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, 
             const R byR = from_right(Wb[BY]);
             const R bzL = from_left(Wb[BZ]);
             const R bzR = from_right(Wb[BZ]);
-            const R dbz_dy = (Wn[BZ] - Wa[BZ]) * inv_2dy;
+            const R dbz_dy = (Whi[BZ] - Wlo[BZ]) * inv_2dy;
             const R dbz_dx = (bzR - bzL) * inv_2dx;
             const R dby_dx = (byR - byL) * inv_2dx;
-            const R dbx_dy = (Wn[BX] - Wa[BX]) * inv_2dy;
+            const R dbx_dy = (Whi[BX] - Wlo[BX]) * inv_2dy;
             Jc[0] = dbz_dy;
             Jc[1] = -dbz_dx;
             Jc[2] = dby_dx - dbx_dy;
         }
-        if (c >= r0) {
-            // y-face c - 1/2 between rows c-1 (Wa) and c (Wb)
+        if (face) {
+            // y-face between rows c - DIR (Wa) and c (Wb), its lower row's state on the left
             R G[kNC];
-            hll_face<1>(WLy, WRy, Wa, Wb, gamma, inv_gm1, c_h, G, ~0ull);
+            if constexpr (DIR > 0) {
+                hll_face<1>(Wc, Wf, Wa, Wb, gamma, inv_gm1, c_h, G, ~0ull);
+            } else {
+                hll_face<1>(Wf, Wc, Wb, Wa, gamma, inv_gm1, c_h, G, ~0ull);
+            }
             if (NI) {
                 // include/mhd/rhs.hpp:96-114
                 R Jf[3], Bf[3], E[3];
@@ -751,9 +780,9 @@ __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, 
                 G[BZ] += -E[0];
                 G[PR] += Bf[2] * E[0] - Bf[0] * E[2];
             }
-            if (c >= r0 + 1) {
-                // x-direction of row c - 1 (primitives Wa), then assemble and update it
-                const int rr = c - 1;
+            if (update) {
+                // x-direction of row c - DIR (primitives Wa), then assemble and update it
+                const int rr = c - DIR;
                 R Wl[kNC], WRx[kNC], WLx[kNC];
 #pragma unroll
                 for (int k = 0; k < kNC; ++k) {
@@ -778,12 +807,15 @@ __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, 
                     F[BZ] += E[1];
                     F[PR] += Bf[1] * E[2] - Bf[2] * E[1];
                 }
+                // the y-fluxes of row rr's upper (rr + 1/2) and lower (rr - 1/2) faces
+                const R (&Gup)[kNC] = DIR > 0 ? G : Gp;
+                const R (&Glo)[kNC] = DIR > 0 ? Gp : G;
                 R L[kNC];
 #pragma unroll
                 for (int k = 0; k < kNC; ++k) {
                     const R Fr = from_right(F[k]);
                     // include/mhd/rhs.hpp:116-123
-                    L[k] = -(Fr - F[k]) * inv_dx - (G[k] - Gp[k]) * inv_dy;
+                    L[k] = -(Fr - F[k]) * inv_dx - (Gup[k] - Glo[k]) * inv_dy;
                 }
                 if (psi_damp != R(0)) {
                     L[PSI] -= psi_damp * Wa[PSI];
@@ -833,7 +865,7 @@ __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, 
         }
 #pragma unroll
         for (int k = 0; k < kNC; ++k) {
-            WLy[k] = WLn[k];
+            Wc[k] = Wn_c[k];
             Wa[k] = Wb[k];
             Wb[k] = Wn[k];
         }
@@ -892,7 +924,7 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageBuffers b, Geom g, Physic
     // wave-private ring of the last three raw (conserved) rows: the RK update of row c-1 reads
     // its stage input from here instead of global memory
     __shared__ R ring[kWavesPerBlock][3][kNC][kWave];
-    const R local_speed = march_tile<STAGE, LIM, NI, REMAP, 0, 1, R>(
+    const R local_speed = march_tile<STAGE, LIM, NI, REMAP, 0, 1, kMarchDir<STAGE>, R>(
         src, u0, dst, b.mask, g, ph, st, strip, r0, r1, 0, r1, ring[wave], lane);
     if (STAGE == 2) {
         const R m = wave_max(local_speed);

@@ -51,6 +51,18 @@ template <typename R>
 __device__ __forceinline__ R vmin(R a, R b) { return (b < a) ? b : a; }
 template <typename R>
 __device__ __forceinline__ R vmax(R a, R b) { return (a < b) ? b : a; }
+// min/max of the flux's wave speeds and discriminants: the select rule above in the exact fp64
+// build; in the fp32 and contracted fp64 builds the hardware v_min / v_max (fused into v_min3 /
+// v_max3 where they chain), which differ from it only in which zero or non-NaN operand they
+// return. Step control (dt, the CFL reductions) keeps the select rule in every build.
+template <typename R>
+__device__ __forceinline__ R hmin(R a, R b) { return vmin(a, b); }
+template <typename R>
+__device__ __forceinline__ R hmax(R a, R b) { return vmax(a, b); }
+template <>
+__device__ __forceinline__ float hmin(float a, float b) { return __builtin_fminf(a, b); }
+template <>
+__device__ __forceinline__ float hmax(float a, float b) { return __builtin_fmaxf(a, b); }
 
 // fp64: correctly rounded sqrt and division (parity with the CPU path).
 // fp64, fast build (MHD_FAST_FP64, its own translation unit, compiled with contraction): the
@@ -62,6 +74,11 @@ __device__ __forceinline__ R vmax(R a, R b) { return (a < b) ? b : a; }
 #define MHD_FAST_FP64 0
 #endif
 #if MHD_FAST_FP64
+template <>
+__device__ __forceinline__ double hmin(double a, double b) { return __builtin_fmin(a, b); }
+template <>
+__device__ __forceinline__ double hmax(double a, double b) { return __builtin_fmax(a, b); }
+
 __device__ __forceinline__ double dsqrt(double x)
 {
     // y ~ 1/sqrt(x); one coupled iteration on g ~ sqrt(x), h ~ 1/(2 sqrt(x)), then the
@@ -140,7 +157,7 @@ struct MixedOps<double> {
     // cf^2 discriminant max(sum^2 - 4 cs2 van2, 0): muscl_hll.hpp:84
     static __device__ __forceinline__ double disc(double sum, double cs2, double van2)
     {
-        return vmax(sum * sum - 4.0 * cs2 * van2, 0.0);
+        return hmax(sum * sum - 4.0 * cs2 * van2, 0.0);
     }
     // p = (gamma-1) (E - ekin - emag) with ekin = 0.5 |m|^2 inv_rho: cpu_rows.hpp:146-153
     static __device__ __forceinline__ double pressure(double rho, double inv_rho, double mx,
@@ -195,7 +212,7 @@ __device__ __forceinline__ R fast_speed(const R (&W)[kNC], R gamma)
     const R sum = cs2 + va2;
     const R disc = MixedOps<R>::disc(sum, cs2, van2);
     const R cf2 = half * (sum + dsqrt(disc));
-    return dsqrt(vmax(cf2, R(0)));
+    return dsqrt(hmax(cf2, R(0)));
 }
 
 // ---- conserved state and GLM-MHD flux along DIR: muscl_hll.hpp:89-124 ---------------------
@@ -278,8 +295,8 @@ __device__ __forceinline__ void hll_face(R (&WL)[kNC], R (&WR)[kNC], const R (&W
     }
     const R cfL = fast_speed<DIR>(WL, gamma);
     const R cfR = fast_speed<DIR>(WR, gamma);
-    const R sL = vmin(vmin(WL[D::N] - cfL, WR[D::N] - cfR), -c_h);
-    const R sR = vmax(vmax(WL[D::N] + cfL, WR[D::N] + cfR), c_h);
+    const R sL = hmin(hmin(WL[D::N] - cfL, WR[D::N] - cfR), -c_h);
+    const R sR = hmax(hmax(WL[D::N] + cfL, WR[D::N] + cfR), c_h);
     const R ch2 = c_h * c_h;
 
     R UL[kNC], FL[kNC], UR[kNC], FR[kNC];
@@ -299,8 +316,8 @@ __device__ __forceinline__ void hll_face(R (&WL)[kNC], R (&WR)[kNC], const R (&W
         const bool degenerate = !use_left && !use_right && (sR - sL) < Lit<R>::kDegenerate;
         // the degenerate lanes weigh FL and FR by 1 / 2 each (speeds -1 and 1) and drop the
         // dissipation term
-        const R sLc = degenerate ? R(-1) : vmin(sL, zero);
-        const R sRc = degenerate ? R(1) : vmax(sR, R(1.17549435e-38f));
+        const R sLc = degenerate ? R(-1) : hmin(sL, zero);
+        const R sRc = degenerate ? R(1) : hmax(sR, R(1.17549435e-38f));
         const R inv_denom = drcp(sRc - sLc);
         const R a = sRc * inv_denom;
         const R b = -sLc * inv_denom;
@@ -364,7 +381,7 @@ __device__ __forceinline__ R cell_max_speed(const R (&U)[kNC], R gamma, R gm1)
     // x: (n, t) = (x, y); y: (n, t) = (y, x) -- the y sum order is by^2 + bx^2 + bz^2
     const R sx = dabs(W[VX]) + fast_speed<0>(W, gamma);
     const R sy = dabs(W[VY]) + fast_speed<1>(W, gamma);
-    return vmax(sx, sy);
+    return hmax(sx, sy);
 }
 
 // ---- non-ideal electric field: nonideal.hpp:63-102 -----------------------------------------

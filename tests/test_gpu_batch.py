@@ -105,7 +105,7 @@ def test_admission_by_hbm_budget(hip):
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
 
 
-@pytest.mark.parametrize("rows", [1, 5, 200])
+@pytest.mark.parametrize("rows", [1, 5, 200, -1])
 def test_batch_tiling_independent(hip, oracle, rows):
     s = mixed_specs(hip, 1, seed=17)[0]
     U0 = s.state()
@@ -144,13 +144,14 @@ def test_batch_nonideal_rectangular_fp64_matches_oracle(hip, oracle, limiter):
         assert np.array_equal(eng.download(i), Uo), (g.nx, g.ny, bc)
 
 
-@pytest.mark.parametrize("rows", [0, 1, 2, 200])
+@pytest.mark.parametrize("rows", [0, 1, 2, 200, -1])
 def test_batch_half_wave_tiles_edges_match_oracle(hip, oracle, rows):
     """Narrow tail strips (at most 28 columns) march on half-wave tiles in the fp64 batch: the
     lower and upper halves of a wave take the same chunk of the strip's lower and upper row
     halves. Edges: tails of 1 and 28 columns, a grid of one tail strip only, 1 to 3 rows (the
     upper half then updates one row or none), odd and even row counts, every boundary mix and
-    rows per wave from 1 to more than the grid (0: the load model's choice)."""
+    rows per wave from 1 to more than the grid (0: the load model's choice; -1: steps taken in
+    turn with the timed shortlist's partitions)."""
     eng = hip.BatchEngine("fp64", limiter=hip.SlopeLimiter.MC, hbm_budget=1 << 28)
     if rows:
         eng.set_seg_rows(rows)
@@ -185,7 +186,7 @@ def test_batch_fp32_narrow_grids_tiling_independent_and_within_l1(hip, oracle):
     states = [oracle.init_ic(name, make_grid(nx, ny, 0.0, 1.0, 0.0, 1.0))
               for name, nx, ny, _ in cases]
     first = None
-    for rows in (0, 1, 2, 200):
+    for rows in (0, 1, 2, 200, -1):
         eng = hip.BatchEngine("fp32", limiter=hip.SlopeLimiter.MC, hbm_budget=1 << 28)
         if rows:
             eng.set_seg_rows(rows)
