@@ -149,12 +149,13 @@ __device__ __forceinline__ bool ctrl_pending(const Ctrl* ctrl, const Physics& ph
 // order its workgroups are dispatched.
 constexpr int kXcds = 8;
 
-__device__ __forceinline__ int xcd_logical_block(int p, int nb)
+__device__ __forceinline__ int xcd_logical_block(int p, int nb, bool reverse = false)
 {
     const int q = nb / kXcds;
     const int r = nb % kXcds;
     const int xcd = p % kXcds;
-    const int local = p / kXcds;
+    const int n = xcd < r ? q + 1 : q;  // blocks on this XCD
+    const int local = reverse ? n - 1 - p / kXcds : p / kXcds;
     return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
 }
 
@@ -909,7 +910,9 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageBuffers b, Geom g, Physic
     // halo columns and the cache lines shared across a strip edge meet in one L2.
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = wave_in_block();
-    const int logical = xcd_logical_block(blockIdx.x, gridDim.x);
+    // (stage 2 takes each XCD's blocks in reverse order, so that it starts on the tiles that
+    // stage 1 touched last)
+    const int logical = xcd_logical_block(blockIdx.x, gridDim.x, STAGE == 2);
     const int strip = logical % tl.strips;
     const int seg = (logical / tl.strips) * kWavesPerBlock + wave;
     if (strip >= tl.strips || seg >= tl.segs) {
