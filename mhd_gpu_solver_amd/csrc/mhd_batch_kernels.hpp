@@ -88,9 +88,9 @@ void k_batch_stage(BatchTables tb, int parity)
         if (!st.active) {
             continue;
         }
-        const R* src = static_cast<const R*>(STAGE == 1 ? in.U : in.Us);
-        const R* u0 = static_cast<const R*>(in.U);
-        R* dst = static_cast<R*>(STAGE == 1 ? in.Us : in.U);
+        const Planes<R> src = Planes<R>::make(STAGE == 1 ? in.U : in.Us, in.g.plane);
+        const Planes<R> u0 = Planes<R>::make(in.U, in.g.plane);
+        const Planes<R> dst = Planes<R>::make(STAGE == 1 ? in.Us : in.U, in.g.plane);
         const int dB = tile.r0b < 0 ? 0 : tile.r0b - tile.r0;
         const R local = march_tile<STAGE, LIM, NI, REMAP, 2, 1, kMarchDir<STAGE>, R>(
             src, u0, dst, REMAP ? nullptr : in.mask, in.g, in.ph, st, tile.strip, tile.r0, tile.r1, dB,

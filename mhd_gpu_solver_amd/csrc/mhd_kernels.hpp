@@ -605,6 +605,26 @@ struct StageBuffers {
     int dt_cap;
 };
 
+// Access to the nine component planes of one state array, through a pointer in the global
+// address space (global_load / global_store rather than flat accesses, whose completion the
+// LDS waits of the march would also have to wait for).
+template <typename R>
+struct Planes {
+    using GR = __attribute__((address_space(1))) R;
+    GR* p;
+    long long plane;  // elements per plane
+
+    __device__ __forceinline__ static Planes make(const void* q, long long plane_elems)
+    {
+        return Planes{(GR*)(const_cast<void*>(q)), plane_elems};
+    }
+    __device__ __forceinline__ R ld(int c, long long cell) const { return p[c * plane + cell]; }
+    __device__ __forceinline__ void st(int c, long long cell, R v) const
+    {
+        p[c * plane + cell] = v;
+    }
+};
+
 // Stage 1 marches its rows in increasing order and stage 2 in decreasing order, so that each
 // stage starts on the rows the previous one touched last (still in the memory-side cache).
 template <int STAGE>
@@ -625,13 +645,13 @@ constexpr int kMarchDir = STAGE == 2 ? -1 : 1;
 // (latency overlapped with the faces, nine more live registers); otherwise at the update.
 //
 // DIR: the order in which the rows are visited (see the loop below).
-template <int STAGE, int LIM, int NI, int REMAP, int HALF, int EARLY, int DIR, typename R>
-__device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, R* dst,
+template <int STAGE, int LIM, int NI, int REMAP, int HALF, int EARLY, int DIR, typename R,
+          typename Planes>
+__device__ __forceinline__ R march_tile(const Planes& src, const Planes& u0, const Planes& dst,
                                         const uint8_t* mask, const Geom& g, const Physics& ph,
                                         const StepState& st, int strip, int r0, int r1, int dB,
                                         int r1b, R (&ring)[3][kNC][kWave], int lane)
 {
-    const long long P = g.plane;
 
     const bool halves = HALF == 1 || (HALF == 2 && dB > 0);  // wave-uniform
     const bool upper = halves && lane >= kWave / 2;
@@ -677,7 +697,7 @@ __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, 
         }
 #pragma unroll
         for (int c = 0; c < kNC; ++c) {
-            u[c] = src[c * P + base];
+            u[c] = src.ld(c, base);
         }
     };
 
@@ -731,7 +751,7 @@ __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, 
                 static_cast<long long>(c - DIR) * g.nxt + (gic + lane_off);
 #pragma unroll
             for (int k = 0; k < kNC; ++k) {
-                U0r[k] = u0[k * P + base];
+                U0r[k] = u0.ld(k, base);
             }
         }
         // y slope of row c and the two face states it produces: toward row c - DIR (used now)
@@ -836,19 +856,19 @@ __device__ __forceinline__ R march_tile(const R* __restrict__ src, const R* u0, 
                         // include/mhd/field_ops.hpp:47-53
 #pragma unroll
                         for (int k = 0; k < kNC; ++k) {
-                            dst[k * P + cell] = ring[slot][k][lane] + dt * L[k];
+                            dst.st(k, cell, ring[slot][k][lane] + dt * L[k]);
                         }
                     } else {
                         // include/mhd/field_ops.hpp:55-63
                         R un[kNC];
 #pragma unroll
                         for (int k = 0; k < kNC; ++k) {
-                            const R u0k = early_u0 ? U0r[k] : u0[k * P + cell];
+                            const R u0k = early_u0 ? U0r[k] : u0.ld(k, cell);
                             un[k] = half * (u0k + ring[slot][k][lane]) + half_dt * L[k];
                         }
 #pragma unroll
                         for (int k = 0; k < kNC; ++k) {
-                            dst[k * P + cell] = un[k];
+                            dst.st(k, cell, un[k]);
                         }
                         const bool fluid =
                             mask == nullptr ||
@@ -921,9 +941,9 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageBuffers b, Geom g, Physic
     const int r0 = g.ng + seg * tl.seg_rows;
     const int r1 = min(r0 + tl.seg_rows, g.ng + g.ny);
 
-    const R* __restrict__ src = static_cast<const R*>(b.src);
-    const R* u0 = static_cast<const R*>(b.u0);
-    R* dst = static_cast<R*>(b.dst);
+    const Planes<R> src = Planes<R>::make(b.src, g.plane);
+    const Planes<R> u0 = Planes<R>::make(b.u0, g.plane);
+    const Planes<R> dst = Planes<R>::make(b.dst, g.plane);
     // wave-private ring of the last three raw (conserved) rows: the RK update of row c-1 reads
     // its stage input from here instead of global memory
     __shared__ R ring[kWavesPerBlock][3][kNC][kWave];
