@@ -330,10 +330,10 @@ int mhd_hip_batch_rel_l1(mhd_hip_batch* a, int id_a, mhd_hip_batch* ref, int id_
  * of tiles (instance, strip, row range). 0 (default): fixed segments whose length a load
  * model picks whenever the stepping set changes (it replays the round-robin hand-out of the
  * tiles to the waves for candidate lengths and keeps the one whose busiest wave has the least
- * work; suits a set that changes, as in serving); -1: the first replays are tuning replays,
- * whose steps (real steps of the run) are taken in pairs with one of a shortlist of up to
- * three lengths the load model ranks within 12 % of its best, the second step of each pair
- * timed; once each has a timing the fastest is kept (suits a set that stays the same); > 0: fixed segments of that many rows handed out
+ * work; suits a set that changes, as in serving); -1: before the first replay, up to three
+ * lengths the load model ranks within 12 % of its best are timed on dry steps (stage 2 writes
+ * the scratch state, so no instance advances) and the fastest is kept (suits a set that stays
+ * the same); > 0: fixed segments of that many rows handed out
  * round-robin; < -1: the capacity fill, every strip cut into row pieces of at least -rows rows
  * poured into the waves in order up to a common capacity. Results do not depend on this
  * value. */

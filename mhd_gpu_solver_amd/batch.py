@@ -141,8 +141,8 @@ class BatchEngine:
 
     def set_seg_rows(self, rows: int):
         """Work partition of the batch kernel: 0 (default) rows per tile from the load model
-        at every change of the stepping set, -1 the first steps taken in turn (in timed pairs)
-        with the fixed lengths the load model ranks best and the fastest kept,
+        at every change of the stepping set, -1 the fixed lengths the load model ranks best timed
+        on dry steps (no instance advances) before the first replay and the fastest kept,
         > 0 fixed rows per tile, < -1 the capacity fill with pieces of at least -rows rows.
         Results do not depend on it."""
         _lib.check(self._l.mhd_hip_batch_set_seg_rows(self.handle, int(rows)))

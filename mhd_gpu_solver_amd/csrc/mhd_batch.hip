@@ -200,13 +200,11 @@ struct mhd_hip_batch {
     int sched_policy = MHD_HIP_SCHED_ALL;
     long long sched_cap = 0;  // interior cells stepping per replay under SRPT
     // Work partition: > 0 fixed segments of that many rows, handed out round-robin; 0 the load
-    // model's length at every table build; -1 timed on real steps over a shortlist, then kept;
+    // model's length at every table build; -1 timed on dry steps over a shortlist, then kept;
     // < -1 the capacity fill with pieces of at least -seg_rows rows.
     int seg_rows = 0;
     int tuned_rows = 0;  // the timed choice: > 0 fixed rows, < 0 fill with -tuned_rows min rows
-    std::vector<int> tune_modes;  // the partitions being timed (empty: none yet, or decided)
-    std::vector<float> tune_ms;   // fastest timed step of each (ms)
-    std::vector<int> tune_count;  // timed steps of each
+    int dry_run = 0;     // set while the timed choice launches its dry steps
     int last_rows = 0;  // fixed rows of the current tile list (0: capacity fill)
     int ntiles = 0;
     int blocks = 0;                      // the largest persistent grid of the variants
@@ -760,7 +758,7 @@ struct mhd_hip_batch {
     BatchTables tables(int g) const
     {
         return BatchTables{d_insts, d_ctrl, d_tiles, d_first + static_cast<size_t>(g) * (max_waves + 1),
-                           d_fill, d_fill + 1};
+                           d_fill, d_fill + 1, dry_run};
     }
 
     // One RK stage of the stepping set: the ghost and wall fill of the masked instances
@@ -816,13 +814,13 @@ struct mhd_hip_batch {
         graph_key = launch_key();
     }
 
-    // Timed choice (rows per wave -1). The first replays are tuning replays: their steps are
-    // real steps of every stepping instance (results never depend on the partition), taken in
-    // pairs with one shortlisted partition each, the one timed least often so far; the second
-    // step of a pair, which follows a step with the same partition as in a run, is timed with
-    // events around its two stage launches. Once every candidate has a timing the fastest is
-    // kept for the engine's life. Shortlist: up to three fixed row lengths, those of least
-    // load-model cost (summed over the kernel variants in use) within 12 % of the least.
+    // Timed choice (rows per wave -1), made before the first replay of an engine whose choice
+    // is not made yet: each shortlisted length is timed on dry steps, stage 1 as in a run (it
+    // writes only U*) and stage 2 writing U* in place of U with no control field touched, so no
+    // instance advances and the samples cost nothing but time. Shortlist: up to three fixed row
+    // lengths, those of least load-model cost (summed over the kernel variants in use) within
+    // 12 % of the least. Two sweeps over the shortlist, the second in reverse order, each
+    // candidate one untimed step and three timed; the fastest step decides.
     std::vector<int> shortlist() const
     {
         std::vector<std::pair<long long, int>> cost;
@@ -841,68 +839,42 @@ struct mhd_hip_batch {
     }
 
     template <typename R, int FAST>
-    void tuning_replay()
+    void autotune()
     {
-        if (tune_modes.empty()) {
-            tune_modes = shortlist();
-            tune_ms.assign(tune_modes.size(), 1e30f);
-            tune_count.assign(tune_modes.size(), 0);
-        }
-        std::vector<int> before(static_cast<size_t>(max_instances), 0);
-        std::vector<char> stepping(static_cast<size_t>(max_instances), 0);
-        for (int si = 0; si < max_instances; ++si) {
-            const Slot& s = slots[si];
-            stepping[si] = s.used && !s.finished && !s.paused;
-            before[si] = h_ctrl[si].steps[0];
-        }
-        while (tev.size() < 4 * kSteps) {
+        const std::vector<int> modes = shortlist();
+        const int n = static_cast<int>(modes.size());
+        std::vector<float> best(modes.size(), 1e30f);
+        while (tev.size() < 2) {
             hipEvent_t e;
             check(hipEventCreate(&e), "hipEventCreate");
             tev.push_back(e);
         }
-        constexpr int kPairs = kSteps / 2;
-        std::vector<int> planned(tune_count), pick(kPairs, 0);
-        for (int j = 0; j < kPairs; ++j) {
-            pick[j] = static_cast<int>(std::min_element(planned.begin(), planned.end()) -
-                                       planned.begin());
-            ++planned[pick[j]];
+        dry_run = 1;
+        try {
+            for (int sweep = 0; sweep < 2; ++sweep) {
+                for (int j = 0; j < n; ++j) {
+                    const int i = sweep ? n - 1 - j : j;
+                    build_tables(modes[i]);
+                    launch_stage<R, FAST>(1, 0);
+                    launch_stage<R, FAST>(2, 0);
+                    for (int rep = 0; rep < 3; ++rep) {
+                        check(hipEventRecord(tev[0], stream), "hipEventRecord");
+                        launch_stage<R, FAST>(1, 0);
+                        launch_stage<R, FAST>(2, 0);
+                        check(hipEventRecord(tev[1], stream), "hipEventRecord");
+                        check(hipEventSynchronize(tev[1]), "hipEventSynchronize");
+                        float ms = 0.0f;
+                        check(hipEventElapsedTime(&ms, tev[0], tev[1]), "hipEventElapsedTime");
+                        best[i] = std::min(best[i], ms);
+                    }
+                }
+            }
+        } catch (...) {
+            dry_run = 0;
+            throw;
         }
-        for (int j = 0; j < kPairs; ++j) {
-            build_tables(tune_modes[pick[j]]);  // waits for the previous pair's launches
-            launch_stage<R, FAST>(1, 0);
-            launch_stage<R, FAST>(2, 0);
-            check(hipEventRecord(tev[2 * j], stream), "hipEventRecord");
-            launch_stage<R, FAST>(1, 1);
-            launch_stage<R, FAST>(2, 1);
-            check(hipEventRecord(tev[2 * j + 1], stream), "hipEventRecord");
-        }
-        check(hipMemcpyAsync(h_ctrl, d_ctrl, sizeof(Ctrl) * max_instances, hipMemcpyDeviceToHost,
-                             stream),
-              "hipMemcpyAsync(ctrl)");
-        check(hipStreamSynchronize(stream), "hipStreamSynchronize");
-        std::vector<int> after(static_cast<size_t>(max_instances), 0);
-        for (int si = 0; si < max_instances; ++si) {
-            after[si] = h_ctrl[si].steps[0];
-        }
-        const int live = mhdhip::replay_active_launches(before, after, stepping, kSteps);
-        for (int j = 0; 2 * j + 1 < live; ++j) {
-            float ms = 0.0f;
-            check(hipEventElapsedTime(&ms, tev[2 * j], tev[2 * j + 1]), "hipEventElapsedTime");
-            tune_ms[pick[j]] = std::min(tune_ms[pick[j]], ms);
-            ++tune_count[pick[j]];
-        }
-        if (*std::min_element(tune_count.begin(), tune_count.end()) >= 1) {
-            tuned_rows = tune_modes[std::min_element(tune_ms.begin(), tune_ms.end()) -
-                                    tune_ms.begin()];
-            tune_modes.clear();
-            // the tables and the captured graph of the choice now, so that the first replay
-            // with it does not pay for them
-            build_tables(plan_mode());
-            dirty = false;
-            ensure_graph<R, FAST>();
-            check(hipGraphUpload(graph, stream), "hipGraphUpload");
-            return;
-        }
+        dry_run = 0;
+        tuned_rows = modes[std::min_element(best.begin(), best.end()) - best.begin()];
         dirty = true;
     }
 
@@ -1041,11 +1013,10 @@ struct mhd_hip_batch {
                 last.valid = false;
                 break;
             }
-            note_launch(last);
             if (seg_rows == -1 && tuned_rows == 0) {
-                tuning_replay<R, FAST>();
-                continue;
+                autotune<R, FAST>();
             }
+            note_launch(last);
             if (dirty) {
                 build_tables(plan_mode());
                 dirty = false;
@@ -1334,7 +1305,6 @@ int mhd_hip_batch_set_seg_rows(mhd_hip_batch* b, int rows)
         if (!b) throw InvalidArgument("null batch");
         b->seg_rows = rows;
         b->tuned_rows = 0;
-        b->tune_modes.clear();
         b->dirty = true;
     });
 }

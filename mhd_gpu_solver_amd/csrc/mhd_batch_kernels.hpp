@@ -35,6 +35,7 @@ struct BatchTables {
     const int* first;       // this launch's per-wave offsets into tiles (waves + 1)
     const int* fill_count;  // how many slots fill_list holds
     const int* fill_list;   // masked slots, whose ghosts and solid cells are filled per stage
+    int dry;  // timing pass: stage 2 writes U* instead of U and no control field is touched
 };
 
 // Waves per SIMD each batched stage kernel is compiled for: what its registers allow (fp32
@@ -73,13 +74,13 @@ void k_batch_stage(BatchTables tb, int parity)
         const StepState st = step_state(ctrl, in.ph, parity);
         // the instance's first tile keeps its step bookkeeping (as block 0 does in k_stage)
         const bool lead = tile.strip == 0 && tile.r0 == in.g.ng && lane == 0;
-        if (STAGE == 1 && lead) {
+        if (STAGE == 1 && lead && !tb.dry) {
             ctrl->chmax[parity ^ 1] = 0ull;
             if (!st.active && ctrl_pending(ctrl, in.ph, parity)) {
                 ctrl->bad_dt = 1;
             }
         }
-        if (STAGE == 2 && lead) {
+        if (STAGE == 2 && lead && !tb.dry) {
             const int q = parity ^ 1;
             ctrl->t[q] = st.active ? st.t + st.dt : st.t;
             ctrl->steps[q] = st.active ? st.steps + 1 : st.steps;
@@ -90,12 +91,12 @@ void k_batch_stage(BatchTables tb, int parity)
         }
         const Planes<R> src = Planes<R>::make(STAGE == 1 ? in.U : in.Us, in.g.plane);
         const Planes<R> u0 = Planes<R>::make(in.U, in.g.plane);
-        const Planes<R> dst = Planes<R>::make(STAGE == 1 ? in.Us : in.U, in.g.plane);
+        const Planes<R> dst = Planes<R>::make(STAGE == 1 || tb.dry ? in.Us : in.U, in.g.plane);
         const int dB = tile.r0b < 0 ? 0 : tile.r0b - tile.r0;
         const R local = march_tile<STAGE, LIM, NI, REMAP, 2, 1, kMarchDir<STAGE>, R>(
             src, u0, dst, REMAP ? nullptr : in.mask, in.g, in.ph, st, tile.strip, tile.r0, tile.r1, dB,
             tile.r0b < 0 ? tile.r1 : tile.r1b, ring[wave], lane);
-        if (STAGE == 2) {
+        if (STAGE == 2 && !tb.dry) {
             const R m = wave_max(local);
             if (lane == 0) {
                 atomic_max_speed(&ctrl->chmax[parity ^ 1], static_cast<double>(m));

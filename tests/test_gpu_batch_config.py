@@ -70,8 +70,9 @@ def oracle_case(oracle, case, g, U0, t_end):
     return oracle.solve(g, U0, p, bc=bc, mask=mask, walls=walls)
 
 
-@pytest.mark.parametrize("rows", [0, 3])
+@pytest.mark.parametrize("rows", [0, 3, -1])
 def test_mixed_schemes_and_masks_fp64_bitexact(hip, oracle, rows):
+    """rows -1: the timed choice's dry steps (every variant, the masked fill) change nothing."""
     eng = hip.BatchEngine("fp64", limiter=hip.SlopeLimiter.MC, hbm_budget=1 << 28)
     if rows:
         eng.set_seg_rows(rows)
