@@ -495,7 +495,8 @@ struct mhd_hip_solver {
     }
 
     // Autotuner: times one step's pair of stage kernels for candidate row-segment lengths on
-    // the current state and keeps the fastest per shape and kernel variant. Stage 1 writes U*
+    // the current state (two sweeps over the candidates, the second in reverse order) and keeps
+    // the fastest per shape and kernel variant. Stage 1 writes U*
     // from U; the timed stage 2 reads U* and U and writes U* again, so U is left untouched
     // (U* is scratch until the next stage 1; the control fields the two launches write are
     // rewritten by the step's own stages). Runs after the first CFL reduction of a solve, so
@@ -555,13 +556,15 @@ struct mhd_hip_solver {
             std::sort(cands.begin(), cands.end());
             cands.erase(std::unique(cands.begin(), cands.end()), cands.end());
         }
-        int best = mhdhip::default_seg_rows(grid.nx, grid.ny);
-        float best_ms = 1e30f;
+        cands.erase(std::remove_if(cands.begin(), cands.end(),
+                                   [&](int c) { return c > grid.ny && c != 8; }),
+                    cands.end());
+        std::vector<float> cand_best(cands.size(), 1e30f);
         try {
-            for (int cand : cands) {
-                if (cand > grid.ny && cand != 8) {
-                    continue;
-                }
+            // two sweeps, the second in reverse order, so that a clock ramp favours no end
+            for (size_t k = 0; k < 2 * cands.size(); ++k) {
+                const size_t ci = k < cands.size() ? k : 2 * cands.size() - 1 - k;
+                const int cand = cands[ci];
                 const Tiling tl = mhdhip::make_tiling(grid.nx, grid.ny, cand);
                 // one untimed pair, then the best of three
                 check(mhdhip::launch_stage<R, FAST>(1, limiter, ni, remap, b1, geom, ph, tl, 0, stream),
@@ -583,10 +586,7 @@ struct mhd_hip_solver {
                     check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
                     cand_ms = std::min(cand_ms, ms);
                 }
-                if (cand_ms < best_ms) {
-                    best_ms = cand_ms;
-                    best = cand;
-                }
+                cand_best[ci] = std::min(cand_best[ci], cand_ms);
             }
         } catch (...) {
             (void)hipEventDestroy(e0);
@@ -595,6 +595,10 @@ struct mhd_hip_solver {
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
+        const int best = cands.empty()
+                             ? mhdhip::default_seg_rows(grid.nx, grid.ny)
+                             : cands[std::min_element(cand_best.begin(), cand_best.end()) -
+                                     cand_best.begin()];
         std::lock_guard<std::mutex> lk(g_tune_mutex);
         g_tuned[key] = best;
         return best;
