@@ -586,11 +586,12 @@ __global__ __launch_bounds__(NC * kWave) void k_l1_final(const DD* __restrict__ 
 //
 // One wave owns a strip of 60 interior columns and marches rows [r0, r1). Lane l holds global
 // column gi = ng + 60 s - 2 + l; lanes 2..61 own cells, lanes 2..62 compute the x-face on the
-// left of their column, lanes 0, 1, 63 only supply neighbours. Per row:
+// left of their column, lanes 0, 1, 63 only supply neighbours. Per row step (written for the
+// downward march; the upward one mirrors it, see march_tile):
 //   * the new row c+1 is loaded (coalesced, 64 consecutive cells per component) and turned into
 //     primitives once;
 //   * y: the slope of row c comes from a three-row register window, the face c-1/2 from the
-//     carried left state, and the previous y-flux is carried, so each y-face is built once;
+//     carried face state, and the previous y-flux is carried, so each y-face is built once;
 //   * x (row c-1): neighbours arrive by lane shuffles, each cell's slope is built once, the
 //     left face state is shuffled from lane l-1 and the right flux from lane l+1.
 // Ghost cells are either read from memory (after k_ghost_fill) or, with REMAP, addressed at
