@@ -117,15 +117,16 @@ def measured_traffic(kernel_prefix, elem_bytes, workload, kernel="k_stage", fast
     return None
 
 
-def roofline(kernels, cells, ms_per_step, elem_bytes, name, traffic=None):
+def roofline(kernels, cells, ms_per_step, elem_bytes, name, traffic=None, mask_bytes=0):
     """Dominant stage kernel: algorithmic bytes per launch (stage 1 reads U, writes U*: 18
-    values per interior cell; stage 2 reads U*, U, writes U: 27) over its mean duration."""
-    per_cell = {0: 18 * elem_bytes, 1: 27 * elem_bytes}
+    values per interior cell; stage 2 reads U*, U, writes U: 27; with a mask, its byte per
+    interior cell in each stage) over its mean duration."""
+    per_cell = {0: 18 * elem_bytes + mask_bytes, 1: 27 * elem_bytes + mask_bytes}
     avg = [kernels[i][0] / max(1, kernels[i][1]) for i in (0, 1)]  # ms
     dom = 0 if avg[0] > avg[1] else 1
     bytes_launch = per_cell[dom] * cells
     achieved = bytes_launch / (avg[dom] * 1e-3) / 1e9
-    step_gbs = 45 * elem_bytes * cells / (ms_per_step * 1e-3) / 1e9
+    step_gbs = (45 * elem_bytes + 2 * mask_bytes) * cells / (ms_per_step * 1e-3) / 1e9
     return {
         "bound": "hbm", "kernel": f"{name}<{dom + 1}>", "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -134,6 +135,74 @@ def roofline(kernels, cells, ms_per_step, elem_bytes, name, traffic=None):
         "stage_avg_ms": [round(a, 4) for a in avg],
         "step_achieved_gbs": round(step_gbs, 1), "step_frac": round(step_gbs / HBM_PEAK_GBS, 4),
     }
+
+
+# ---- configured single runs (Orszag-Tang 256^2 to t = 0.5; rotor 1024^2 non-ideal + walls) ---
+
+def config_leg(m, U0, n, precision, p, bc=(0, 0), mask=None, steps=None):
+    """One configured solve on the single-run solver, on the graph path between stream
+    synchronisations with the state resident on the device (uploaded before the timed region):
+    `steps` steps, or to p.t_end. Before it, an untimed pass (autotuning of the shape, graph
+    capture; for a solve to t_end it also gives the step count) and the same step count with
+    events around every stage kernel. Returns the device solver too, holding the final state."""
+    from mhd_gpu_solver_amd import _lib
+    grid = _lib.Grid(n, n, 2, 0.0, 1.0, 0.0, 1.0)
+    dev = m.DeviceSolver(grid, _lib.PRECISIONS[precision])
+    dev.configure(bc[0], bc[1], mask, True)
+
+    def start():
+        dev.upload(U0)
+        dev.fill_boundaries(p.gamma)
+
+    start()
+    count = steps if steps else dev.solve(p).steps   # warm-up (a whole solve to t_end)
+    if steps:
+        dev.advance_steps(p, min(steps, 8))
+    start()
+    dev.set_kernel_timing(True)
+    dev.advance_steps(p, count)
+    k = dev.kernel_times()
+    dev.set_kernel_timing(False)
+    start()
+    t0 = time.perf_counter()
+    r = dev.advance_steps(p, steps) if steps else dev.solve(p)
+    secs = time.perf_counter() - t0
+    cells = n * n
+    return dict(mcups=cells * r.steps / secs / 1e6, ms_per_step=secs * 1e3 / r.steps,
+                steps=r.steps, t=r.t, kernels=k, cells=cells, seg_rows=dev.seg_rows(), dev=dev)
+
+
+def configs_leg(m, workloads):
+    """Orszag-Tang 256^2 (B0 = 1/sqrt(4 pi), minmod, periodic) to t = 0.5, and the rotor 1024^2
+    with Ohmic, Hall and ambipolar terms, outflow and 8 seeded solid circles as reflective walls,
+    200 steps: fp64 and FP32, the per-field relative L1 of FP32 against fp64 on the device."""
+    out = {}
+    p = params(m)
+    p.t_end = 0.5
+    U0 = m.init_ic("orszag_tang_b0", 256, 256, gamma=5.0 / 3.0, b0=B0_STD)
+    legs = {prec: config_leg(m, U0, 256, prec, p) for prec in ("fp64", "fp32")}
+    out["orszag_tang_256_t0.5"] = _config_summary(m, legs, mask_bytes=0)
+    n = 1024
+    spec = workloads.rotor_circles(n)
+    p = params(m)
+    p.nonideal = spec.nonideal
+    legs = {prec: config_leg(m, spec.state(), n, prec, p, bc=spec.bc, mask=spec.mask, steps=200)
+            for prec in ("fp64", "fp32")}
+    out["rotor_1024_nonideal_8_circles_200_steps"] = _config_summary(m, legs, mask_bytes=1)
+    return out
+
+
+def _config_summary(m, legs, mask_bytes):
+    res = {}
+    for prec, leg in legs.items():
+        eb = 8 if prec == "fp64" else 4
+        res[prec] = {"value": round(leg["mcups"], 2), "ms_per_step": round(leg["ms_per_step"], 4),
+                     "steps": leg["steps"], "t": leg["t"], "seg_rows": leg["seg_rows"],
+                     "roofline": roofline(leg["kernels"], leg["cells"], leg["ms_per_step"], eb,
+                                          "k_stage", mask_bytes=mask_bytes)}
+    l1 = legs["fp32"]["dev"].rel_l1(legs["fp64"]["dev"])
+    res["fp32_rel_l1_vs_fp64"] = {f: float(f"{v:.3e}") for f, v in zip(m.FIELDS[:8], l1[:8])}
+    return res
 
 
 # ---- ensemble -------------------------------------------------------------------------------
@@ -333,14 +402,17 @@ def main():
     ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["ensemble", "single", "compare", "serve"], default=None)
+    ap.add_argument("--only", choices=["ensemble", "single", "compare", "serve", "configs"],
+                    default=None)
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the reference's backend-comparison table (OT, MC, t_end 0.02)")
     ap.add_argument("--serve-requests", type=int, default=128)
     ap.add_argument("--batch-rows", type=int, default=-1,
-                    help="batch rows per wave: -1 timed on the first steps (default: the "
+                    help="batch rows per wave: -1 timed on dry steps (default: the "
                          "instance set is static), 0 load model, >0 fixed")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the Orszag-Tang 256^2 and rotor 1024^2 non-ideal + walls runs")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="also run plain device copies of BYTES (counter scale for profiling)")
     args = ap.parse_args()
@@ -472,6 +544,18 @@ def main():
             out.update({"value": legs["fp64"]["value"], "ms_per_step": legs["fp64"]["ms_per_step"],
                         "dtype": "f64", "config": {"workload": f"orszag_tang_{n}x{n}_single_run"},
                         "roofline": legs["fp64"]["roofline"], "cpu_baseline": None})
+    if not args.no_configs and args.only in (None, "configs"):
+        log("configured runs: Orszag-Tang 256^2 to t = 0.5; rotor 1024^2 non-ideal + walls")
+        cfg = configs_leg(m, workloads)
+        for name, res in cfg.items():
+            log(f"{name}: fp64 {res['fp64']['value']} fp32 {res['fp32']['value']} Mcell-updates/s, "
+                f"{res['fp64']['steps']} steps")
+        out["configs"] = cfg
+        if args.only == "configs":
+            top = cfg["rotor_1024_nonideal_8_circles_200_steps"]["fp32"]
+            out.update({"value": top["value"], "dtype": "f32", "ms_per_step": top["ms_per_step"],
+                        "config": {"workload": "rotor_1024_nonideal_8_circles_200_steps"},
+                        "roofline": top["roofline"], "cpu_baseline": None})
     if not args.no_compare and args.only in (None, "compare"):
         log("backend comparison: Orszag-Tang (B0 = 1), MC, t_end 0.02, transfers included")
         cmp_ = backend_compare(m, (128, 256, 512, 1024), 0.02)
