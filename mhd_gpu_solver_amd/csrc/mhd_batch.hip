@@ -179,7 +179,7 @@ struct mhd_hip_batch {
     BatchInst* d_insts = nullptr;
     Ctrl* d_ctrl = nullptr;
     BatchTile* d_tiles = nullptr;
-    int* d_first = nullptr;  // per kernel variant, per-wave tile offsets (max_waves + 1 each)
+    int* d_first = nullptr;  // per kernel variant and stage, per-wave tile offsets (max_waves + 1 each)
     int* d_fill = nullptr;   // count, then the slots filled before each stage
     size_t tile_cap = 0;
 
@@ -208,7 +208,7 @@ struct mhd_hip_batch {
     int last_rows = 0;  // fixed rows of the current tile list (0: capacity fill)
     int ntiles = 0;
     int blocks = 0;                      // the largest persistent grid of the variants
-    int group_blocks[kGroups] = {};      // persistent grid of each kernel variant
+    int group_blocks[kGroups][2] = {};   // persistent grid of each kernel variant, per stage
     int max_waves = 0;
     unsigned used_groups = 0;            // variants with stepping instances (bit g)
     bool fill_needed = false;            // stepping instances with a mask
@@ -341,17 +341,19 @@ struct mhd_hip_batch {
         for (int g = 0; g < kGroups; ++g) {
             const int lim = group_limiter(g);
             const bool ni = group_nonideal(g), remap = group_remap(g);
-            const int per_cu =
-                precision == MHD_HIP_FP64
-                    ? mhdhip::batch_blocks_per_cu<double, 0>(lim, ni, remap)
-                    : precision == MHD_HIP_FP64_FAST
-                          ? mhdhip::batch_blocks_per_cu<double, 1>(lim, ni, remap)
-                          : mhdhip::batch_blocks_per_cu<float, 0>(lim, ni, remap);
-            group_blocks[g] = std::max(1, per_cu) * prop.multiProcessorCount;
-            blocks = std::max(blocks, group_blocks[g]);
+            for (int st = 1; st <= 2; ++st) {
+                const int per_cu =
+                    precision == MHD_HIP_FP64
+                        ? mhdhip::batch_blocks_per_cu<double, 0>(st, lim, ni, remap)
+                        : precision == MHD_HIP_FP64_FAST
+                              ? mhdhip::batch_blocks_per_cu<double, 1>(st, lim, ni, remap)
+                              : mhdhip::batch_blocks_per_cu<float, 0>(st, lim, ni, remap);
+                group_blocks[g][st - 1] = std::max(1, per_cu) * prop.multiProcessorCount;
+                blocks = std::max(blocks, group_blocks[g][st - 1]);
+            }
         }
         max_waves = blocks * mhdhip::kWavesPerBlock;
-        const size_t nfirst = static_cast<size_t>(kGroups) * (max_waves + 1);
+        const size_t nfirst = static_cast<size_t>(kGroups) * 2 * (max_waves + 1);
         check(hipMalloc(reinterpret_cast<void**>(&d_first), sizeof(int) * nfirst),
               "hipMalloc(first)");
         check(hipMalloc(reinterpret_cast<void**>(&d_fill), sizeof(int) * (max_inst + 1)),
@@ -654,7 +656,11 @@ struct mhd_hip_batch {
     // Row lengths the load model weighs (and the timed choice shortlists from).
     static constexpr int kModelRowCandidates[] = {8, 12, 16, 20, 24, 28, 32, 40, 48, 64, 96, 128};
 
-    int group_waves(int g) const { return group_blocks[g] * mhdhip::kWavesPerBlock; }
+    // waves of variant g's persistent grid in stage st (1 or 2)
+    int group_waves(int g, int st) const
+    {
+        return group_blocks[g][st - 1] * mhdhip::kWavesPerBlock;
+    }
 
     // The stepping instances of kernel variant g.
     std::vector<mhdhip::PlanInst> stepping_set(int g) const
@@ -687,29 +693,36 @@ struct mhd_hip_batch {
         return 0;
     }
 
-    // Tables of the stepping set: every slot's BatchInst; per kernel variant, its tiles and
-    // per-wave offsets (variant g's offsets at g * (max_waves + 1)); the list of instances
-    // whose ghosts and solid cells are filled before each stage.
+    // Tables of the stepping set: every slot's BatchInst; per kernel variant and stage, its
+    // tiles and per-wave offsets (variant g's stage st offsets at (2 g + st - 1) (max_waves + 1));
+    // the list of instances whose ghosts and solid cells are filled before each stage. The two
+    // stages of a variant share one tile list when their grids are the same size.
     void build_tables(int mode)
     {
         std::vector<BatchTile> tiles;
-        std::vector<int> first(static_cast<size_t>(kGroups) * (max_waves + 1), 0);
+        std::vector<int> first(static_cast<size_t>(kGroups) * 2 * (max_waves + 1), 0);
         used_groups = 0;
         last_rows = 0;
         for (int g = 0; g < kGroups; ++g) {
             const std::vector<mhdhip::PlanInst> insts = stepping_set(g);
-            const int base = static_cast<int>(tiles.size());
-            int* f = first.data() + static_cast<size_t>(g) * (max_waves + 1);
-            std::fill(f, f + max_waves + 1, base);
-            if (insts.empty()) continue;
-            const int waves = group_waves(g);
-            const int r = resolve(mode, insts, waves);
-            const mhdhip::WavePlan p = r < 0 ? mhdhip::plan_fill(insts, waves, -r)
-                                             : mhdhip::plan_fixed(insts, r, waves);
-            for (int w = 0; w <= waves; ++w) f[w] = base + p.first[static_cast<size_t>(w)];
-            tiles.insert(tiles.end(), p.tiles.begin(), p.tiles.end());
-            used_groups |= 1u << g;
-            if (last_rows == 0) last_rows = r > 0 ? r : 0;
+            for (int st = 1; st <= 2; ++st) {
+                const int base = static_cast<int>(tiles.size());
+                int* f = first.data() + static_cast<size_t>(2 * g + st - 1) * (max_waves + 1);
+                std::fill(f, f + max_waves + 1, base);
+                if (insts.empty()) continue;
+                const int waves = group_waves(g, st);
+                if (st == 2 && waves == group_waves(g, 1)) {
+                    std::copy(f - (max_waves + 1), f, f);  // stage 1's list
+                    continue;
+                }
+                const int r = resolve(mode, insts, waves);
+                const mhdhip::WavePlan p = r < 0 ? mhdhip::plan_fill(insts, waves, -r)
+                                                 : mhdhip::plan_fixed(insts, r, waves);
+                for (int w = 0; w <= waves; ++w) f[w] = base + p.first[static_cast<size_t>(w)];
+                tiles.insert(tiles.end(), p.tiles.begin(), p.tiles.end());
+                if (last_rows == 0 || st == 2) last_rows = r > 0 ? r : 0;
+            }
+            if (!insts.empty()) used_groups |= 1u << g;
         }
         int nfill = 0;
         const size_t n = tiles.size();
@@ -755,9 +768,10 @@ struct mhd_hip_batch {
               "hipMemcpyAsync(fill)");
     }
 
-    BatchTables tables(int g) const
+    BatchTables tables(int g, int st) const
     {
-        return BatchTables{d_insts, d_ctrl, d_tiles, d_first + static_cast<size_t>(g) * (max_waves + 1),
+        return BatchTables{d_insts, d_ctrl, d_tiles,
+                           d_first + static_cast<size_t>(2 * g + st - 1) * (max_waves + 1),
                            d_fill, d_fill + 1, dry_run};
     }
 
@@ -767,14 +781,15 @@ struct mhd_hip_batch {
     void launch_stage(int stage, int parity)
     {
         if (fill_needed) {
-            check(mhdhip::launch_batch_fill<R>(tables(0), parity, stage, max_instances, stream),
+            check(mhdhip::launch_batch_fill<R>(tables(0, stage), parity, stage, max_instances,
+                                               stream),
                   "batch fill");
         }
         for (int g = 0; g < kGroups; ++g) {
             if (used_groups & (1u << g)) {
                 check(mhdhip::launch_batch_stage<R, FAST>(stage, group_limiter(g), group_nonideal(g),
-                                                    group_remap(g), tables(g), parity,
-                                                    group_blocks[g], stream),
+                                                    group_remap(g), tables(g, stage), parity,
+                                                    group_blocks[g][stage - 1], stream),
                       stage == 1 ? "batch stage1" : "batch stage2");
             }
         }
@@ -828,7 +843,10 @@ struct mhd_hip_batch {
         for (int g = 0; g < kGroups; ++g) {
             const std::vector<mhdhip::PlanInst> insts = stepping_set(g);
             if (insts.empty()) continue;
-            for (auto& c : cost) c.first += mhdhip::fixed_max_cost(insts, c.second, group_waves(g));
+            for (auto& c : cost) {
+                c.first += mhdhip::fixed_max_cost(insts, c.second, group_waves(g, 1)) +
+                           mhdhip::fixed_max_cost(insts, c.second, group_waves(g, 2));
+            }
         }
         std::sort(cost.begin(), cost.end());
         std::vector<int> modes;

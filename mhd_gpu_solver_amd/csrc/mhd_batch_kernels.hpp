@@ -39,10 +39,12 @@ struct BatchTables {
 };
 
 // Waves per SIMD each batched stage kernel is compiled for: what its registers allow (fp32
-// ideal 128 VGPRs, non-ideal 168; fp64 ideal 256, non-ideal 512 with AGPRs). The fp32 ideal
-// kernel needs the bound: its march for both tile kinds lands one register above 128.
-template <typename R, int NI>
-constexpr int kBatchWavesPerEU = sizeof(R) == 4 ? (NI ? 3 : 4) : (NI ? 1 : 2);
+// ideal 128 VGPRs, non-ideal 168; fp64 ideal and non-ideal stage 1 256, non-ideal stage 2 512
+// with AGPRs: at 256 it would spill 64-88 bytes per lane). The fp32 ideal kernel needs the
+// bound: its march for both tile kinds lands one register above 128. Each stage has its own
+// persistent grid, so the two stages of one variant may run at different occupancies.
+template <typename R, int NI, int STAGE>
+constexpr int kBatchWavesPerEU = sizeof(R) == 4 ? (NI ? 3 : 4) : (NI && STAGE == 2 ? 1 : 2);
 
 // REMAP 1: instances without solid cells; ghosts are addressed at their interior sources.
 // REMAP 0: instances with a mask; ghosts (and, with reflective walls, the solid cells) are
@@ -51,7 +53,7 @@ constexpr int kBatchWavesPerEU = sizeof(R) == 4 ? (NI ? 3 : 4) : (NI ? 1 : 2);
 // right-hand side and keeps them out of the CFL speed.
 template <int STAGE, int LIM, int NI, int REMAP, typename R, int FAST = 0>
 __global__ __launch_bounds__(kBlock)
-__attribute__((amdgpu_waves_per_eu(kBatchWavesPerEU<R, NI>)))
+__attribute__((amdgpu_waves_per_eu(kBatchWavesPerEU<R, NI, STAGE>)))
 void k_batch_stage(BatchTables tb, int parity)
 {
     static_assert(FAST == MHD_FAST_FP64, "stepping kernel built in the wrong translation unit");

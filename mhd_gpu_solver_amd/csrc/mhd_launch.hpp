@@ -87,8 +87,8 @@ template <typename R>
 hipError_t launch_batch_fill(const BatchTables& tb, int parity, int stage, int slots,
                              hipStream_t s);
 
-// Workgroups of k_batch_stage that fit on one CU at once (occupancy query).
+// Workgroups of k_batch_stage of this stage that fit on one CU at once (occupancy query).
 template <typename R, int FAST = 0>
-int batch_blocks_per_cu(int limiter, bool nonideal, bool remap);
+int batch_blocks_per_cu(int stage, int limiter, bool nonideal, bool remap);
 
 }  // namespace mhdhip

@@ -152,29 +152,35 @@ hipError_t launch_batch_fill(const BatchTables& tb, int parity, int stage, int s
 
 namespace detail {
 
-template <int LIM, int NI, typename R, int FAST>
+template <int STAGE, int LIM, int NI, typename R, int FAST>
 hipError_t batch_occupancy(bool remap, int* n)
 {
     return remap ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                       n, k_batch_stage<2, LIM, NI, 1, R, FAST>, kBlock, 0)
+                       n, k_batch_stage<STAGE, LIM, NI, 1, R, FAST>, kBlock, 0)
                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                       n, k_batch_stage<2, LIM, NI, 0, R, FAST>, kBlock, 0);
+                       n, k_batch_stage<STAGE, LIM, NI, 0, R, FAST>, kBlock, 0);
+}
+
+template <int STAGE, typename R, int FAST>
+hipError_t batch_occupancy_lim(int limiter, bool nonideal, bool remap, int* n)
+{
+    if (limiter == kMinmod) {
+        return nonideal ? batch_occupancy<STAGE, kMinmod, 1, R, FAST>(remap, n)
+                        : batch_occupancy<STAGE, kMinmod, 0, R, FAST>(remap, n);
+    }
+    return nonideal ? batch_occupancy<STAGE, kMC, 1, R, FAST>(remap, n)
+                    : batch_occupancy<STAGE, kMC, 0, R, FAST>(remap, n);
 }
 
 }  // namespace detail
 
 template <typename R, int FAST>
-int batch_blocks_per_cu(int limiter, bool nonideal, bool remap)
+int batch_blocks_per_cu(int stage, int limiter, bool nonideal, bool remap)
 {
     int n = 0;
-    hipError_t e;
-    if (limiter == kMinmod) {
-        e = nonideal ? detail::batch_occupancy<kMinmod, 1, R, FAST>(remap, &n)
-                     : detail::batch_occupancy<kMinmod, 0, R, FAST>(remap, &n);
-    } else {
-        e = nonideal ? detail::batch_occupancy<kMC, 1, R, FAST>(remap, &n)
-                     : detail::batch_occupancy<kMC, 0, R, FAST>(remap, &n);
-    }
+    const hipError_t e =
+        stage == 1 ? detail::batch_occupancy_lim<1, R, FAST>(limiter, nonideal, remap, &n)
+                   : detail::batch_occupancy_lim<2, R, FAST>(limiter, nonideal, remap, &n);
     return (e == hipSuccess && n > 0) ? n : 1;
 }
 
@@ -248,7 +254,7 @@ int stage_blocks_per_cu(int limiter, bool nonideal, bool remap)
     template hipError_t launch_batch_stage<R>(int, int, bool, bool, const BatchTables&, int, int, \
                                               hipStream_t);                                       \
     template hipError_t launch_batch_fill<R>(const BatchTables&, int, int, int, hipStream_t);      \
-    template int batch_blocks_per_cu<R>(int, bool, bool);                                        \
+    template int batch_blocks_per_cu<R>(int, int, bool, bool);                                   \
     template int stage_blocks_per_cu<R>(int, bool, bool);                                        \
     template hipError_t launch_totals<R>(const R*, const Geom&, DD*, double*, hipStream_t);     \
     template hipError_t launch_max_div_b<R>(const R*, const Geom&, double, double,              \
@@ -263,7 +269,7 @@ int stage_blocks_per_cu(int limiter, bool nonideal, bool remap)
                                            hipStream_t);                                         \
     template hipError_t launch_batch_stage<R, 1>(int, int, bool, bool, const BatchTables&, int,  \
                                                  int, hipStream_t);                              \
-    template int batch_blocks_per_cu<R, 1>(int, bool, bool);                                     \
+    template int batch_blocks_per_cu<R, 1>(int, int, bool, bool);                                \
     template int stage_blocks_per_cu<R, 1>(int, bool, bool);
 
 }  // namespace mhdhip
