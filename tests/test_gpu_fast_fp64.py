@@ -1,7 +1,6 @@
 """The fast fp64 build (precision "fp64_fast": contracted multiply-adds, Newton-refined
-hardware reciprocal and square root in the stepping kernels) against the oracle at the
-north star's bar: within 1e-12 relative L1 per conserved field after a fixed step count at the
-same dt sequence (the oracle driven by the GPU's logged dt: orc_solve_dt; the GLM scalar psi,
+hardware reciprocal and square root in the stepping kernels) against the oracle: within 1e-12
+relative L1 per conserved field after a fixed step count at the same dt sequence (the oracle driven by the GPU's logged dt: orc_solve_dt; the GLM scalar psi,
 which stays at round-off size, is bounded against |B| instead). Also the
 reference's fp64 GPU gate (tools/mhd_backend_compare.cpp:242-250: equal step counts, relative
 L1 <= 1e-11, L-inf <= 1e-9, drift <= 1e-11) and the batched engine, masks and non-ideal terms
