@@ -544,7 +544,7 @@ def main():
             out.update({"value": legs["fp64"]["value"], "ms_per_step": legs["fp64"]["ms_per_step"],
                         "dtype": "f64", "config": {"workload": f"orszag_tang_{n}x{n}_single_run"},
                         "roofline": legs["fp64"]["roofline"], "cpu_baseline": None})
-    if not args.no_configs and args.only in (None, "configs"):
+    if not args.no_configs and args.only == "configs":
         log("configured runs: Orszag-Tang 256^2 to t = 0.5; rotor 1024^2 non-ideal + walls")
         cfg = configs_leg(m, workloads)
         for name, res in cfg.items():
@@ -556,6 +556,21 @@ def main():
             out.update({"value": top["value"], "dtype": "f32", "ms_per_step": top["ms_per_step"],
                         "config": {"workload": "rotor_1024_nonideal_8_circles_200_steps"},
                         "roofline": top["roofline"], "cpu_baseline": None})
+    if not args.no_configs and args.only is None:
+        # the configured runs in a child process of their own: a failure there (reported in
+        # the line) cannot take the line with it
+        import subprocess
+        log("configured runs (child process)")
+        try:
+            child = subprocess.run([sys.executable, str(pathlib.Path(__file__).resolve()),
+                                    "--only", "configs"], capture_output=True, text=True,
+                                   timeout=600)
+            sys.stderr.write(child.stderr)
+            out["configs"] = json.loads(child.stdout.strip().splitlines()[-1])["configs"]
+        except subprocess.TimeoutExpired:
+            out["configs"] = {"error": "timed out"}
+        except (IndexError, KeyError, ValueError):
+            out["configs"] = {"error": f"exit status {child.returncode}"}
     if not args.no_compare and args.only in (None, "compare"):
         log("backend comparison: Orszag-Tang (B0 = 1), MC, t_end 0.02, transfers included")
         cmp_ = backend_compare(m, (128, 256, 512, 1024), 0.02)
