@@ -179,7 +179,7 @@ struct mhd_hip_batch {
     BatchInst* d_insts = nullptr;
     Ctrl* d_ctrl = nullptr;
     BatchTile* d_tiles = nullptr;
-    int* d_first = nullptr;  // per kernel variant and stage, per-wave tile offsets (max_waves + 1 each)
+    int* d_first = nullptr;  // per variant and stage, per-wave tile offsets (max_waves + 1 each)
     int* d_fill = nullptr;   // count, then the slots filled before each stage
     size_t tile_cap = 0;
 
