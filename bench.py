@@ -210,7 +210,8 @@ def _config_summary(m, legs, mask_bytes):
 def ensemble_batched(m, specs, states, precision, steps, warmup, seg_rows=0):
     """The whole ensemble in one batch engine: `warmup` steps on each launch path (eager with
     events, then the graph: tiling, capture), then `steps` steps timed with events around every
-    stage kernel, then the same on the graph path, whose rate is the leg's value."""
+    stage kernel, then the same on the graph path, whose rate is the leg's value. Each timed
+    region starts after the admission and the planning of the admitted set (`prepare`)."""
     budget = sum(m.instance_bytes(precision, s.n, s.n) for s in specs) + (64 << 20)
     eng = m.BatchEngine(precision, limiter=m.SlopeLimiter.Minmod, hbm_budget=budget,
                         max_instances=len(specs))
@@ -231,6 +232,7 @@ def ensemble_batched(m, specs, states, precision, steps, warmup, seg_rows=0):
     admit_s = time.perf_counter() - t0
     before = {i: eng.totals(i) for i, s in zip(ids, specs) if s.bc == (0, 0)}
     eng.set_kernel_timing(True)
+    eng.prepare()                                    # the admitted set's tile tables
     t0 = time.perf_counter()
     eng.advance()
     wall = time.perf_counter() - t0
@@ -247,6 +249,9 @@ def ensemble_batched(m, specs, states, precision, steps, warmup, seg_rows=0):
         eng.remove(i)
     ids = admit(steps)
     t0 = time.perf_counter()
+    eng.prepare()                                    # the admitted set's tables and graph
+    prepare_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
     eng.advance()
     graph_s = time.perf_counter() - t0
     for i in ids:
@@ -255,7 +260,7 @@ def ensemble_batched(m, specs, states, precision, steps, warmup, seg_rows=0):
     return dict(mcups=cells * steps / graph_s / 1e6, ms_per_step=graph_s * 1e3 / steps,
                 timed_mcups=cells * steps / wall / 1e6, kernels=kernels, cells=cells,
                 seg_rows=stats["seg_rows"], blocks=stats["blocks"], results=results,
-                admit_s=admit_s,
+                admit_s=admit_s, prepare_s=prepare_s,
                 drift={"mass": float(f"{drift[0]:.3e}"), "energy": float(f"{drift[1]:.3e}"),
                        "instances": len(before)})
 
@@ -484,6 +489,7 @@ def main():
             "row_choice": {-1: "timed", 0: "load model"}.get(args.batch_rows, "fixed"),
             "drift_fp32_periodic_max": b32["drift"],
             "admission_s": round(b32["admit_s"], 4),
+            "planning_s": round(b32["prepare_s"], 4),
             "batched_fp64_fast": {"value": round(b64f["mcups"], 2),
                                   "ms_per_step": round(b64f["ms_per_step"], 4),
                                   "seg_rows": b64f["seg_rows"],

@@ -286,6 +286,13 @@ int mhd_hip_batch_remove(mhd_hip_batch* b, int id);
  * no progress would repeat forever). */
 int mhd_hip_batch_advance(mhd_hip_batch* b, int max_replays, int* active);
 
+/* Does before the next replay what its launch needs (no counterpart in the reference, whose
+ * solver plans nothing across calls): reads the control blocks, schedules the stepping set,
+ * makes the timed row choice if it is pending, builds the tile tables and captures the step
+ * graph. mhd_hip_batch_advance does this itself when needed; a caller that admitted a burst of
+ * instances can call this first so that the planning is not part of the next advance. */
+int mhd_hip_batch_prepare(mhd_hip_batch* b);
+
 /* Result so far (t, steps, c_h as SolveResult) and whether the instance has finished; the
  * ::solve() errors are reported here (EINVAL for dt <= 0, ERUNTIME for max_steps before t_end). */
 int mhd_hip_batch_result(mhd_hip_batch* b, int id, mhd_hip_result* result, int* finished);

@@ -95,6 +95,7 @@ SIGNATURES = [
                                            C.POINTER(C.c_int)]),
     ("mhd_hip_batch_remove", C.c_int, [C.c_void_p, C.c_int]),
     ("mhd_hip_batch_advance", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
+    ("mhd_hip_batch_prepare", C.c_int, [C.c_void_p]),
     ("mhd_hip_batch_result", C.c_int, [C.c_void_p, C.c_int, C.POINTER(Result),
                                        C.POINTER(C.c_int)]),
     ("mhd_hip_batch_download", C.c_int, [C.c_void_p, C.c_int, DP9, C.c_size_t]),

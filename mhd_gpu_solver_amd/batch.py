@@ -99,6 +99,11 @@ class BatchEngine:
         _lib.check(self._l.mhd_hip_batch_remove(self.handle, int(iid)))
         self._shape.pop(iid, None)
 
+    def prepare(self):
+        """Plan the next replay now (tile tables, timed row choice, step graph), so that the
+        next advance starts with its launch."""
+        _lib.check(self._l.mhd_hip_batch_prepare(self.handle))
+
     def advance(self, max_replays: int = -1) -> int:
         """Step until no instance is active (or max_replays graph replays); returns active."""
         a = C.c_int()

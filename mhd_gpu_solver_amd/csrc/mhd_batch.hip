@@ -1060,6 +1060,43 @@ struct mhd_hip_batch {
         return active;
     }
 
+    // What the next replay needs before its launch, done now: the control blocks read, the
+    // stepping set scheduled, the timed row choice made, the tile tables built and the step
+    // graph captured. The next advance then starts with the launch.
+    template <typename R, int FAST>
+    void prepare_t()
+    {
+        check(hipSetDevice(device), "hipSetDevice");
+        read_ctrl();
+        schedule();
+        bool any = false;
+        for (const auto& s : slots) {
+            any = any || (s.used && !s.finished && !s.paused);
+        }
+        if (!any) {
+            return;
+        }
+        if (seg_rows == -1 && tuned_rows == 0) {
+            autotune<R, FAST>();
+        }
+        if (dirty) {
+            build_tables(plan_mode());
+            dirty = false;
+        }
+        if (!kernel_timing) {
+            ensure_graph<R, FAST>();
+        }
+    }
+
+    void prepare()
+    {
+        switch (precision) {
+            case MHD_HIP_FP64: prepare_t<double, 0>(); break;
+            case MHD_HIP_FP64_FAST: prepare_t<double, 1>(); break;
+            default: prepare_t<float, 0>(); break;
+        }
+    }
+
     int advance(int max_replays)
     {
         switch (precision) {
@@ -1269,6 +1306,14 @@ int mhd_hip_batch_advance(mhd_hip_batch* b, int max_replays, int* active)
     return guarded([&] {
         if (!b || !active) throw InvalidArgument("null argument");
         *active = b->advance(max_replays);
+    });
+}
+
+int mhd_hip_batch_prepare(mhd_hip_batch* b)
+{
+    return guarded([&] {
+        if (!b) throw InvalidArgument("null batch");
+        b->prepare();
     });
 }
 

@@ -105,6 +105,28 @@ def test_admission_by_hbm_budget(hip):
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
 
 
+@pytest.mark.parametrize("rows", [0, -1])
+def test_batch_prepare_then_advance_matches_oracle(hip, oracle, rows):
+    """prepare() plans the next replay (tables, timed row choice, graph) without stepping: on
+    an empty engine it does nothing, called twice it changes nothing, and the solves that
+    follow equal the oracle bit for bit (state, steps, t)."""
+    eng = hip.BatchEngine("fp64", limiter=hip.SlopeLimiter.MC, hbm_budget=1 << 30)
+    eng.set_seg_rows(rows)
+    eng.prepare()
+    specs = mixed_specs(hip, 4, seed=29)
+    runs = [(eng.add(s.state(), s.t_end, gamma=s.gamma, bc=s.bc), s) for s in specs]
+    eng.prepare()
+    eng.prepare()
+    for i, s in runs:
+        assert eng.result(i)[0].steps == 0
+    assert eng.advance() == 0
+    for i, s in runs:
+        Uo, info = oracle_run(oracle, s, s.state(), MC)
+        r, fin = eng.result(i)
+        assert fin and r.steps == info["steps"] and r.t == info["t"]
+        assert np.array_equal(eng.download(i), Uo)
+
+
 @pytest.mark.parametrize("rows", [1, 5, 200, -1])
 def test_batch_tiling_independent(hip, oracle, rows):
     s = mixed_specs(hip, 1, seed=17)[0]
