@@ -838,22 +838,17 @@ struct mhd_hip_batch {
     // candidate one untimed step and three timed; the fastest step decides.
     std::vector<int> shortlist() const
     {
-        std::vector<std::pair<long long, int>> cost;
-        for (const int rows : kModelRowCandidates) cost.emplace_back(0, rows);
+        std::vector<std::vector<mhdhip::PlanInst>> sets;
+        for (int g = 0; g < kGroups; ++g) sets.push_back(stepping_set(g));
+        std::vector<mhdhip::PlanLaunch> launches;
         for (int g = 0; g < kGroups; ++g) {
-            const std::vector<mhdhip::PlanInst> insts = stepping_set(g);
-            if (insts.empty()) continue;
-            for (auto& c : cost) {
-                c.first += mhdhip::fixed_max_cost(insts, c.second, group_waves(g, 1)) +
-                           mhdhip::fixed_max_cost(insts, c.second, group_waves(g, 2));
+            for (int st = 1; st <= 2; ++st) {
+                launches.push_back(mhdhip::PlanLaunch{&sets[g], group_waves(g, st)});
             }
         }
-        std::sort(cost.begin(), cost.end());
-        std::vector<int> modes;
-        for (size_t i = 0; i < cost.size() && i < 3; ++i) {
-            if (cost[i].first * 100 <= cost.front().first * 112) modes.push_back(cost[i].second);
-        }
-        return modes;
+        return mhdhip::shortlist_rows(launches,
+                                      std::vector<int>(std::begin(kModelRowCandidates),
+                                                       std::end(kModelRowCandidates)));
     }
 
     template <typename R, int FAST>

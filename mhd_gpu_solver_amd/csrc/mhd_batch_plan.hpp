@@ -155,6 +155,35 @@ inline int model_rows(const std::vector<PlanInst>& insts, int waves,
     return best;
 }
 
+// One persistent launch as the load model sees it: the instances it steps and its wave count.
+struct PlanLaunch {
+    const std::vector<PlanInst>* insts;
+    int waves;
+};
+
+// The timed row choice's candidates: the fixed row lengths of least load-model cost (the
+// busiest wave's cost, summed over `launches`), at most `count` of them and within `pct` % of
+// the least, cheapest first (equal costs: the longer length first).
+inline std::vector<int> shortlist_rows(const std::vector<PlanLaunch>& launches,
+                                       const std::vector<int>& candidates, int count = 3,
+                                       int pct = 12)
+{
+    std::vector<std::pair<long long, int>> cost;
+    for (const int rows : candidates) {
+        long long c = 0;
+        for (const PlanLaunch& l : launches) {
+            if (!l.insts->empty()) c += fixed_max_cost(*l.insts, rows, l.waves);
+        }
+        cost.emplace_back(c, -rows);
+    }
+    std::sort(cost.begin(), cost.end());
+    std::vector<int> out;
+    for (size_t i = 0; i < cost.size() && static_cast<int>(out.size()) < count; ++i) {
+        if (cost[i].first * 100 <= cost.front().first * (100 + pct)) out.push_back(-cost[i].second);
+    }
+    return out;
+}
+
 // Capacity fill: the strips of all instances, in instance order, are cut into row pieces and
 // poured into the waves in order, each wave filled up to a common capacity C (half-rows,
 // piece overheads included), so every wave carries about the same work in one round of the

@@ -1,10 +1,12 @@
 // CPU checks of the batched engine's host planning (mhd_gpu_solver_amd/csrc/mhd_batch_plan.hpp):
 // every interior cell of every instance is marched exactly once, each instance has exactly one
 // lead tile (its step bookkeeping), the per-wave offsets are consistent, the capacity fill
-// balances the waves, and the replay accounting / progress check behave on fake step counts.
+// balances the waves, the timed choice's shortlist ranks by the load model, and the replay
+// accounting / progress check behave on fake step counts.
 // Built and run by tests/test_batch_plan.py; prints "ok" and exits 0 when every check holds.
 #include "mhd_batch_plan.hpp"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <random>
@@ -133,6 +135,35 @@ int main()
                 best = best < 0 ? c : std::min(best, c);
             }
             CHECK(fixed_max_cost(set, pick, waves) == best, "model pick %d", pick);
+        }
+    }
+    // the timed choice's shortlist: at most 3 lengths, cheapest first, all within 12 % of the
+    // cheapest, which is the load model's pick; summed over launches; empty sets cost nothing
+    for (const auto& set : sets) {
+        const std::vector<int> cands = {8, 12, 16, 20, 24, 28, 32, 40, 48, 64, 96, 128};
+        const std::vector<PlanInst> none;
+        for (const int waves : {64, 2048, 4096}) {
+            const std::vector<int> sl =
+                shortlist_rows({PlanLaunch{&set, waves}, PlanLaunch{&none, waves}}, cands);
+            CHECK(!sl.empty() && sl.size() <= 3, "shortlist size %zu", sl.size());
+            if (sl.empty()) continue;
+            const long long c0 = fixed_max_cost(set, sl[0], waves);
+            CHECK(sl[0] == model_rows(set, waves, cands), "shortlist head %d", sl[0]);
+            for (size_t i = 0; i < sl.size(); ++i) {
+                const long long c = fixed_max_cost(set, sl[i], waves);
+                CHECK(c * 100 <= c0 * 112, "shortlist %d beyond 12 %%", sl[i]);
+                if (i > 0) CHECK(c >= fixed_max_cost(set, sl[i - 1], waves), "shortlist order");
+            }
+            for (const int r : cands) {  // nothing cheaper left out
+                const bool in = std::find(sl.begin(), sl.end(), r) != sl.end();
+                if (!in && sl.size() < 3) {
+                    CHECK(fixed_max_cost(set, r, waves) * 100 > c0 * 112, "left out %d", r);
+                }
+            }
+            // two launches of the same set: the same ranking
+            const std::vector<int> two =
+                shortlist_rows({PlanLaunch{&set, waves}, PlanLaunch{&set, waves}}, cands);
+            CHECK(two == sl, "two equal launches rank alike");
         }
     }
     // replay accounting

@@ -1,6 +1,7 @@
 """CPU checks of the batched engine's host planning (csrc/mhd_batch_plan.hpp): tile coverage,
-one lead tile per instance, per-wave offsets, the capacity fill's balance, and the replay
-accounting / no-progress check, compiled with g++ from tests/host/batch_plan_check.cpp."""
+one lead tile per instance, per-wave offsets, the capacity fill's balance, the timed row
+choice's shortlist, and the replay accounting / no-progress check, compiled with g++ from
+tests/host/batch_plan_check.cpp."""
 import pathlib
 import shutil
 import subprocess
