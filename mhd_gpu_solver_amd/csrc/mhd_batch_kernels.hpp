@@ -91,13 +91,20 @@ void k_batch_stage(BatchTables tb, int parity)
         if (!st.active) {
             continue;
         }
-        const Planes<R> src = Planes<R>::make(STAGE == 1 ? in.U : in.Us, in.g.plane);
-        const Planes<R> u0 = Planes<R>::make(in.U, in.g.plane);
-        const Planes<R> dst = Planes<R>::make(STAGE == 1 || tb.dry ? in.Us : in.U, in.g.plane);
-        const int dB = tile.r0b < 0 ? 0 : tile.r0b - tile.r0;
+        // the instance's geometry and the tile's rows in scalar registers (they were read by
+        // vector loads): the march forms its row addresses from them
+        const Geom g = uniform_geom(in.g);
+        const Planes<R> src = Planes<R>::make(STAGE == 1 ? in.U : in.Us, g.plane);
+        const Planes<R> u0 = Planes<R>::make(in.U, g.plane);
+        const Planes<R> dst = Planes<R>::make(STAGE == 1 || tb.dry ? in.Us : in.U, g.plane);
+        const int r0 = uniform_i32(tile.r0);
+        const int r1 = uniform_i32(tile.r1);
+        const int dB = uniform_i32(tile.r0b < 0 ? 0 : tile.r0b - tile.r0);
+        const int r1b = uniform_i32(tile.r0b < 0 ? tile.r1 : tile.r1b);
+        const int strip = uniform_i32(tile.strip);
         const R local = march_tile<STAGE, LIM, NI, REMAP, 2, 1, kMarchDir<STAGE>, R>(
-            src, u0, dst, REMAP ? nullptr : in.mask, in.g, in.ph, st, tile.strip, tile.r0, tile.r1, dB,
-            tile.r0b < 0 ? tile.r1 : tile.r1b, ring[wave], lane);
+            src, u0, dst, REMAP ? nullptr : in.mask, g, in.ph, st, strip, r0, r1, dB, r1b,
+            ring[wave], lane);
         if (STAGE == 2 && !tb.dry) {
             const R m = wave_max(local);
             if (lane == 0) {

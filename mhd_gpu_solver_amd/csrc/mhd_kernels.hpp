@@ -98,6 +98,41 @@ struct Geom {
     int bc_x, bc_y;   // 0 periodic, 2 outflow
 };
 
+// A 64-bit value from its two 32-bit words. Both words are unsigned, so that the low word
+// cannot sign-extend into the high one.
+__host__ __device__ constexpr unsigned long long join_words(unsigned hi, unsigned lo)
+{
+    return (static_cast<unsigned long long>(hi) << 32) | static_cast<unsigned long long>(lo);
+}
+static_assert(join_words(0x1u, 0x80000000u) == 0x180000000ull &&
+                  join_words(0x7fffu, 0xfffffff0u) == 0x7ffffffffff0ull,
+              "a low word with bit 31 set stays in the low half");
+
+// A wave-uniform value read into scalar registers (readfirstlane returns an int per word).
+__device__ __forceinline__ int uniform_i32(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ unsigned long long uniform_u64(unsigned long long x)
+{
+    return join_words(
+        static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(x >> 32))),
+        static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(x & 0xffffffffu))));
+}
+
+// A copy of g read into scalar registers (g is wave-uniform but may have been read by vector
+// loads).
+__device__ __forceinline__ Geom uniform_geom(const Geom& g)
+{
+    Geom u;
+    u.nx = uniform_i32(g.nx);
+    u.ny = uniform_i32(g.ny);
+    u.ng = uniform_i32(g.ng);
+    u.nxt = uniform_i32(g.nxt);
+    u.nyt = uniform_i32(g.nyt);
+    u.plane = static_cast<long long>(uniform_u64(static_cast<unsigned long long>(g.plane)));
+    u.bc_x = uniform_i32(g.bc_x);
+    u.bc_y = uniform_i32(g.bc_y);
+    return u;
+}
+
 struct Physics {
     double gamma, gm1, inv_gm1;
     double cfl, inv_dx, inv_dy, inv_2dx, inv_2dy;
@@ -608,21 +643,46 @@ struct StageBuffers {
 
 // Access to the nine component planes of one state array, through a pointer in the global
 // address space (global_load / global_store rather than flat accesses, whose completion the
-// LDS waits of the march would also have to wait for).
+// LDS waits of the march would also have to wait for). An access names a wave-uniform row
+// (its address is formed in scalar registers) and a lane's unsigned byte offset from it, so
+// that it compiles to a global_load / global_store with a scalar base and a 32-bit vector
+// offset: no per-access 64-bit address arithmetic on the vector ALU.
 template <typename R>
 struct Planes {
     using GR = __attribute__((address_space(1))) R;
+    using GB = __attribute__((address_space(1))) char;
     GR* p;
     long long plane;  // elements per plane
 
+    // (q and plane_elems are wave-uniform; they are read into scalar registers here, so that
+    // values a caller loaded through vector loads may feed the scalar row addresses below)
     __device__ __forceinline__ static Planes make(const void* q, long long plane_elems)
     {
-        return Planes{(GR*)(const_cast<void*>(q)), plane_elems};
+        const unsigned long long a = uniform_u64(reinterpret_cast<unsigned long long>(q));
+        const unsigned long long n = uniform_u64(static_cast<unsigned long long>(plane_elems));
+        return Planes{reinterpret_cast<GR*>(a), static_cast<long long>(n)};
     }
-    __device__ __forceinline__ R ld(int c, long long cell) const { return p[c * plane + cell]; }
-    __device__ __forceinline__ void st(int c, long long cell, R v) const
+    // element `boff` bytes past the start of row r of component c (row length nxt). The row
+    // address passes through an empty asm statement that pins it to scalar registers, and the
+    // offset through one that keeps it a 32-bit vector register at the access: the compiler
+    // can then neither fold the lane's offset into the sum of plane and row offsets (it would
+    // form every component's address on the vector ALU) nor widen it to 64 bits away from the
+    // access, and the access takes the scalar-base form. (No instruction is emitted for them.)
+    __device__ __forceinline__ GR& at(int c, int r, int nxt, unsigned boff) const
     {
-        p[c * plane + cell] = v;
+        GR* row = p + (c * plane + static_cast<long long>(r) * nxt);
+        unsigned long long a = reinterpret_cast<unsigned long long>(row);
+        asm("" : "+s"(a));
+        asm("" : "+v"(boff));
+        return *reinterpret_cast<GR*>(reinterpret_cast<GB*>(a) + boff);
+    }
+    __device__ __forceinline__ R ld(int c, int r, int nxt, unsigned boff) const
+    {
+        return at(c, r, nxt, boff);
+    }
+    __device__ __forceinline__ void st(int c, int r, int nxt, unsigned boff, R v) const
+    {
+        at(c, r, nxt, boff) = v;
     }
 };
 
@@ -635,15 +695,20 @@ constexpr int kMarchDir = STAGE == 2 ? -1 : 1;
 // RK stage to its cells. `ring` is this wave's LDS slice. Returns the wave-local maximum CFL
 // signal speed of the updated fluid cells (stage 2; 0 in stage 1).
 //
-// HALF 1: the strip is a narrow one (at most kHalfCells columns, the tail of a grid) and the
-// wave works as two half-wave strips of 32 lanes, each with its own 2 halo lanes on either
-// side. Lanes 0..31 march rows [r0, r1), lanes 32..63 rows [r0 + dB, r1 + dB), of which only
-// those below r1b are updated. Shifts across lanes 31/32 feed only halo lanes, whose faces are
-// never used (the same argument as for the wave's edge lanes). HALF 2: either kind, chosen per
-// tile by dB > 0 (wave-uniform), with one copy of the march for both; HALF 0: whole-wave only.
+// HALF 2: a tile with dB > 0 (wave-uniform) is a narrow strip (at most kHalfCells columns,
+// the tail of a grid) and the wave works as two half-wave strips of 32 lanes, each with its
+// own 2 halo lanes on either side. Lanes 0..31 march rows [r0, r1), lanes 32..63 rows
+// [r0 + dB, r1 + dB), of which only those below r1b are updated. Shifts across lanes 31/32
+// feed only halo lanes, whose faces are never used (the same argument as for the wave's edge
+// lanes). A tile with dB = 0 is a whole-wave strip. HALF 0: whole-wave strips only.
 //
 // EARLY: stage 2 issues its loads of U for the updated row at the start of the row step
 // (latency overlapped with the faces, nine more live registers); otherwise at the update.
+//
+// Every row step issues the same loads (the rows are clamped where the march runs out), U
+// first and the next row second: the wait before the update then covers only the loads of U,
+// and the wait for the next row at the top of the following step lets the stores of the
+// update stay in flight.
 //
 // DIR: the order in which the rows are visited (see the loop below).
 template <int STAGE, int LIM, int NI, int REMAP, int HALF, int EARLY, int DIR, typename R,
@@ -653,8 +718,8 @@ __device__ __forceinline__ R march_tile(const Planes& src, const Planes& u0, con
                                         const StepState& st, int strip, int r0, int r1, int dB,
                                         int r1b, R (&ring)[3][kNC][kWave], int lane)
 {
-
-    const bool halves = HALF == 1 || (HALF == 2 && dB > 0);  // wave-uniform
+    static_assert(HALF == 0 || HALF == 2, "whole-wave tiles, or both kinds");
+    const bool halves = HALF == 2 && dB > 0;  // wave-uniform
     const bool upper = halves && lane >= kWave / 2;
     const int l = halves ? (lane & (kWave / 2 - 1)) : lane;  // lane within its strip
     const int row_off = upper ? dB : 0;                        // this lane's row offset
@@ -666,6 +731,11 @@ __device__ __forceinline__ R march_tile(const Planes& src, const Planes& u0, con
     const int col = REMAP ? ghost_source(gic, g.nx, g.ng, g.bc_x) : gic;
     const bool cell_lane =
         l >= 2 && l < 2 + (halves ? kHalfCells : kStripCells) && gi < g.ng + g.nx;
+    // byte offsets of this lane's column in a row it loads, and of its cell in the row it
+    // updates (lane_off included), from the start of the row
+    const unsigned col_boff = static_cast<unsigned>(col) * static_cast<unsigned>(sizeof(R));
+    const unsigned cell_boff =
+        static_cast<unsigned>(gic + lane_off) * static_cast<unsigned>(sizeof(R));
     const unsigned long long x_live = halves ? kXFaceLanesHalf : kXFaceLanes;
 
     const R gamma = static_cast<R>(ph.gamma);
@@ -685,20 +755,25 @@ __device__ __forceinline__ R march_tile(const Planes& src, const Planes& u0, con
     const R half_dt = static_cast<R>(0.5 * st.dt);
 
     auto load_row = [&](int q, R (&u)[kNC]) {
-        long long base;
+        int qbase;
+        unsigned boff = col_boff;
         if constexpr (HALF == 2) {
-            // both halves' rows are remapped as wave-uniform values; a lane selects its own
+            // both halves' rows are wave-uniform values; the lower one is the base and a lane
+            // adds its own row's offset. Without the remap the upper half's last chunk of an
+            // odd ny reaches one row past the grid, a row whose values feed only faces of rows
+            // that are not updated: it is clamped to the last ghost row.
             const int qa = REMAP ? ghost_source(q, g.ny, g.ng, g.bc_y) : q;
-            const int qb = REMAP ? ghost_source(q + dB, g.ny, g.ng, g.bc_y) : q + dB;
-            base = static_cast<long long>(qa) * g.nxt + (col + (upper ? (qb - qa) * g.nxt : 0));
+            const int qb = REMAP ? ghost_source(q + dB, g.ny, g.ng, g.bc_y)
+                                 : min(q + dB, g.nyt - 1);
+            qbase = min(qa, qb);
+            boff += static_cast<unsigned>(((upper ? qb : qa) - qbase) * g.nxt) *
+                    static_cast<unsigned>(sizeof(R));
         } else {
-            const int ql = q + row_off;
-            const int qr = REMAP ? ghost_source(ql, g.ny, g.ng, g.bc_y) : ql;
-            base = static_cast<long long>(qr) * g.nxt + col;
+            qbase = REMAP ? ghost_source(q, g.ny, g.ng, g.bc_y) : q;
         }
 #pragma unroll
         for (int c = 0; c < kNC; ++c) {
-            u[c] = src.ld(c, base);
+            u[c] = src.ld(c, qbase, g.nxt, boff);
         }
     };
 
@@ -736,25 +811,24 @@ __device__ __forceinline__ R march_tile(const Planes& src, const Planes& u0, con
         const bool update = DIR > 0 ? c >= r0 + 1 : c <= r1 - 2;
         ring_put(c + DIR, Uraw);
         to_prim(Uraw, gm1, Wn);  // row c + DIR
-        if (DIR > 0 ? c + 2 <= r1 + 1 : c - 2 >= r0 - 2) {
-            load_row(c + 2 * DIR, Uraw);
+        // stage 2 updates row c - DIR from U at the start of the step: with EARLY those loads
+        // are issued now, so that their latency overlaps the face computations (column clamped
+        // like the row loads; a step with no update loads a row of the tile that it does not
+        // use)
+        constexpr bool early_u0 = STAGE == 2 && EARLY;
+        R U0r[kNC];
+        if constexpr (early_u0) {
+            const int ru = DIR > 0 ? max(c - DIR, r0) : min(c - DIR, r1 - 1);
+#pragma unroll
+            for (int k = 0; k < kNC; ++k) {
+                U0r[k] = u0.ld(k, ru, g.nxt, cell_boff);
+            }
         }
+        // row c + 2 DIR for the next step (the last step reloads row c + DIR, unused)
+        load_row(DIR > 0 ? min(c + 2, r1 + 1) : max(c - 2, r0 - 2), Uraw);
         // the rows below and above row c
         const R (&Wlo)[kNC] = DIR > 0 ? Wa : Wn;
         const R (&Whi)[kNC] = DIR > 0 ? Wn : Wa;
-        // stage 2 updates row c - DIR from U at the start of the step: with EARLY those loads
-        // are issued now, so that their latency overlaps the face computations (column clamped
-        // like the row loads)
-        constexpr bool early_u0 = STAGE == 2 && EARLY;
-        R U0r[kNC];
-        if (early_u0 && update) {
-            const long long base =
-                static_cast<long long>(c - DIR) * g.nxt + (gic + lane_off);
-#pragma unroll
-            for (int k = 0; k < kNC; ++k) {
-                U0r[k] = u0.ld(k, base);
-            }
-        }
         // y slope of row c and the two face states it produces: toward row c - DIR (used now)
         // and toward row c + DIR (carried)
         R Wf[kNC], Wn_c[kNC];
@@ -851,25 +925,25 @@ __device__ __forceinline__ R march_tile(const Planes& src, const Planes& u0, con
                             L[k] = R(0);
                         }
                     }
-                    const long long cell = static_cast<long long>(rr) * g.nxt + (gi + lane_off);
+                    // (a cell lane's column is in the grid, so gic = gi here)
                     const int slot = slot_of(rr);
                     if (STAGE == 1) {
                         // include/mhd/field_ops.hpp:47-53
 #pragma unroll
                         for (int k = 0; k < kNC; ++k) {
-                            dst.st(k, cell, ring[slot][k][lane] + dt * L[k]);
+                            dst.st(k, rr, g.nxt, cell_boff, ring[slot][k][lane] + dt * L[k]);
                         }
                     } else {
                         // include/mhd/field_ops.hpp:55-63
                         R un[kNC];
 #pragma unroll
                         for (int k = 0; k < kNC; ++k) {
-                            const R u0k = early_u0 ? U0r[k] : u0.ld(k, cell);
+                            const R u0k = early_u0 ? U0r[k] : u0.ld(k, rr, g.nxt, cell_boff);
                             un[k] = half * (u0k + ring[slot][k][lane]) + half_dt * L[k];
                         }
 #pragma unroll
                         for (int k = 0; k < kNC; ++k) {
-                            dst.st(k, cell, un[k]);
+                            dst.st(k, rr, g.nxt, cell_boff, un[k]);
                         }
                         const bool fluid =
                             mask == nullptr ||
