@@ -181,6 +181,11 @@ struct mhd_hip_batch {
     BatchTile* d_tiles = nullptr;
     int* d_first = nullptr;  // per variant and stage, per-wave tile offsets (max_waves + 1 each)
     int* d_fill = nullptr;   // count, then the slots filled before each stage
+    // queued hand-out: per variant and stage, the queue offsets into the tiles (kQueueSlots
+    // each), and the queue counters (kPlanXcds each)
+    static constexpr int kQueueSlots = mhdhip::kPlanXcds + 1;
+    int* d_queue = nullptr;
+    int* d_next = nullptr;
     size_t tile_cap = 0;
 
     BatchInst* h_insts = nullptr;  // pinned staging
@@ -188,6 +193,7 @@ struct mhd_hip_batch {
     BatchTile* h_tiles = nullptr;
     int* h_first = nullptr;
     int* h_fill = nullptr;
+    int* h_queue = nullptr;
     size_t h_tile_cap = 0;
     double* h_stage = nullptr;  // pinned staging for state upload/download
     size_t h_stage_bytes = 0;
@@ -204,6 +210,12 @@ struct mhd_hip_batch {
     // < -1 the capacity fill with pieces of at least -seg_rows rows.
     int seg_rows = 0;
     int tuned_rows = 0;  // the timed choice: > 0 fixed rows, < 0 fill with -tuned_rows min rows
+    bool queued = false;  // the current tables hand tiles out from per-XCD queues
+    // The queue counters are zeroed by the stage kernels themselves (each stage zeroes the
+    // other's as it starts), which holds while every launch of stage 1 is followed by one of
+    // stage 2. A launch sequence cut short by an error leaves them unknown: the host zeroes
+    // them before the next launches.
+    bool counters_known = true;
     int dry_run = 0;     // set while the timed choice launches its dry steps
     int last_rows = 0;  // fixed rows of the current tile list (0: capacity fill)
     int ntiles = 0;
@@ -234,11 +246,14 @@ struct mhd_hip_batch {
         if (d_tiles) (void)hipFree(d_tiles);
         if (d_first) (void)hipFree(d_first);
         if (d_fill) (void)hipFree(d_fill);
+        if (d_queue) (void)hipFree(d_queue);
+        if (d_next) (void)hipFree(d_next);
         if (h_insts) (void)hipHostFree(h_insts);
         if (h_ctrl) (void)hipHostFree(h_ctrl);
         if (h_tiles) (void)hipHostFree(h_tiles);
         if (h_first) (void)hipHostFree(h_first);
         if (h_fill) (void)hipHostFree(h_fill);
+        if (h_queue) (void)hipHostFree(h_queue);
         if (h_stage) (void)hipHostFree(h_stage);
         if (totals_partial) (void)hipFree(totals_partial);
         if (totals_out) (void)hipFree(totals_out);
@@ -269,6 +284,7 @@ struct mhd_hip_batch {
             check(hipEventCreate(&e), "hipEventCreate");
             tev.push_back(e);
         }
+        open_launches();
         for (int k = 0; k < kSteps; ++k) {
             const int p = k & 1;
             check(hipEventRecord(tev[4 * k], stream), "hipEventRecord");
@@ -278,6 +294,7 @@ struct mhd_hip_batch {
             launch_stage<R, FAST>(2, p);
             check(hipEventRecord(tev[4 * k + 3], stream), "hipEventRecord");
         }
+        counters_known = true;
         check(hipMemcpyAsync(h_ctrl, d_ctrl, sizeof(Ctrl) * max_instances, hipMemcpyDeviceToHost,
                              stream),
               "hipMemcpyAsync(ctrl)");
@@ -358,6 +375,19 @@ struct mhd_hip_batch {
               "hipMalloc(first)");
         check(hipMalloc(reinterpret_cast<void**>(&d_fill), sizeof(int) * (max_inst + 1)),
               "hipMalloc(fill)");
+        const size_t nqueue = static_cast<size_t>(kGroups) * 2 * kQueueSlots;
+        const size_t nnext = static_cast<size_t>(kGroups) * 2 * mhdhip::kPlanXcds;
+        check(hipMalloc(reinterpret_cast<void**>(&d_queue), sizeof(int) * nqueue),
+              "hipMalloc(queue)");
+        check(hipMalloc(reinterpret_cast<void**>(&d_next), sizeof(int) * nnext),
+              "hipMalloc(next)");
+        check(hipHostMalloc(reinterpret_cast<void**>(&h_queue), sizeof(int) * nqueue, 0),
+              "hipHostMalloc");
+        std::fill(h_queue, h_queue + nqueue, 0);
+        check(hipMemcpyAsync(d_queue, h_queue, sizeof(int) * nqueue, hipMemcpyHostToDevice,
+                             stream),
+              "hipMemcpyAsync");
+        check(hipMemsetAsync(d_next, 0, sizeof(int) * nnext, stream), "hipMemsetAsync(next)");
         check(hipHostMalloc(reinterpret_cast<void**>(&h_insts), sizeof(BatchInst) * max_inst, 0),
               "hipHostMalloc");
         check(hipHostMalloc(reinterpret_cast<void**>(&h_ctrl), sizeof(Ctrl) * max_inst, 0),
@@ -676,14 +706,29 @@ struct mhd_hip_batch {
     }
 
     // The partition a mode stands for on a stepping set: > 0 fixed segments of that many rows
-    // handed out round-robin (0: the load model's length for the set); < 0 the capacity fill
-    // with pieces of at least -mode rows.
+    // in per-XCD queues (0: the load model's length for the set); < 0 the capacity fill with
+    // pieces of at least -mode rows, as static per-wave lists.
     static int resolve(int mode, const std::vector<mhdhip::PlanInst>& insts, int waves)
     {
         if (mode != 0) return mode;
-        return mhdhip::model_rows(insts, waves,
-                                  std::vector<int>(std::begin(kModelRowCandidates),
-                                                   std::end(kModelRowCandidates)));
+        return mhdhip::model_rows_for(insts, waves,
+                                      std::vector<int>(std::begin(kModelRowCandidates),
+                                                       std::end(kModelRowCandidates)),
+                                      mhdhip::HandOut::kQueued);
+    }
+
+    // Before a sequence of stage launches: zero the queue counters if an earlier sequence was
+    // cut short (counters_known is set again once the sequence is enqueued whole).
+    void open_launches()
+    {
+        if (!counters_known) {
+            check(hipMemsetAsync(d_next, 0,
+                                 sizeof(int) * static_cast<size_t>(kGroups) * 2 *
+                                     mhdhip::kPlanXcds,
+                                 stream),
+                  "hipMemsetAsync(next)");
+        }
+        counters_known = false;
     }
 
     int plan_mode() const
@@ -694,32 +739,48 @@ struct mhd_hip_batch {
     }
 
     // Tables of the stepping set: every slot's BatchInst; per kernel variant and stage, its
-    // tiles and per-wave offsets (variant g's stage st offsets at (2 g + st - 1) (max_waves + 1));
-    // the list of instances whose ghosts and solid cells are filled before each stage. The two
-    // stages of a variant share one tile list when their grids are the same size.
+    // tiles, as per-XCD queues (offsets at (2 g + st - 1) kQueueSlots) for fixed row lengths or
+    // as per-wave lists (offsets at (2 g + st - 1) (max_waves + 1)) for the capacity fill; the
+    // list of instances whose ghosts and solid cells are filled before each stage. The two
+    // stages of a variant share one tile list when their grids split it alike.
     void build_tables(int mode)
     {
         std::vector<BatchTile> tiles;
         std::vector<int> first(static_cast<size_t>(kGroups) * 2 * (max_waves + 1), 0);
+        std::vector<int> qbegin(static_cast<size_t>(kGroups) * 2 * kQueueSlots, 0);
         used_groups = 0;
         last_rows = 0;
+        queued = false;
         for (int g = 0; g < kGroups; ++g) {
             const std::vector<mhdhip::PlanInst> insts = stepping_set(g);
             for (int st = 1; st <= 2; ++st) {
                 const int base = static_cast<int>(tiles.size());
                 int* f = first.data() + static_cast<size_t>(2 * g + st - 1) * (max_waves + 1);
+                int* qb = qbegin.data() + static_cast<size_t>(2 * g + st - 1) * kQueueSlots;
                 std::fill(f, f + max_waves + 1, base);
+                std::fill(qb, qb + kQueueSlots, base);
                 if (insts.empty()) continue;
                 const int waves = group_waves(g, st);
-                if (st == 2 && waves == group_waves(g, 1)) {
-                    std::copy(f - (max_waves + 1), f, f);  // stage 1's list
-                    continue;
-                }
                 const int r = resolve(mode, insts, waves);
-                const mhdhip::WavePlan p = r < 0 ? mhdhip::plan_fill(insts, waves, -r)
-                                                 : mhdhip::plan_fixed(insts, r, waves);
-                for (int w = 0; w <= waves; ++w) f[w] = base + p.first[static_cast<size_t>(w)];
-                tiles.insert(tiles.end(), p.tiles.begin(), p.tiles.end());
+                if (r > 0) {
+                    const int nq = mhdhip::launch_queues(group_blocks[g][st - 1]);
+                    if (st == 2 && nq == mhdhip::launch_queues(group_blocks[g][0])) {
+                        std::copy(qb - kQueueSlots, qb, qb);  // stage 1's queues
+                    } else {
+                        const mhdhip::QueuePlan p = mhdhip::plan_queues(insts, r, nq);
+                        for (int x = 0; x <= nq; ++x) qb[x] = base + p.begin[static_cast<size_t>(x)];
+                        tiles.insert(tiles.end(), p.tiles.begin(), p.tiles.end());
+                    }
+                    queued = true;
+                } else {
+                    if (st == 2 && waves == group_waves(g, 1)) {
+                        std::copy(f - (max_waves + 1), f, f);  // stage 1's list
+                        continue;
+                    }
+                    const mhdhip::WavePlan p = mhdhip::plan_fill(insts, waves, -r);
+                    for (int w = 0; w <= waves; ++w) f[w] = base + p.first[static_cast<size_t>(w)];
+                    tiles.insert(tiles.end(), p.tiles.begin(), p.tiles.end());
+                }
                 if (last_rows == 0 || st == 2) last_rows = r > 0 ? r : 0;
             }
             if (!insts.empty()) used_groups |= 1u << g;
@@ -751,6 +812,7 @@ struct mhd_hip_batch {
         fill_needed = nfill > 0;
         std::copy(tiles.begin(), tiles.end(), h_tiles);
         std::copy(first.begin(), first.end(), h_first);
+        std::copy(qbegin.begin(), qbegin.end(), h_queue);
         ntiles = static_cast<int>(n);
         check(hipMemcpyAsync(d_insts, h_insts, sizeof(BatchInst) * max_instances,
                              hipMemcpyHostToDevice, stream),
@@ -763,6 +825,9 @@ struct mhd_hip_batch {
         check(hipMemcpyAsync(d_first, h_first, sizeof(int) * first.size(), hipMemcpyHostToDevice,
                              stream),
               "hipMemcpyAsync(first)");
+        check(hipMemcpyAsync(d_queue, h_queue, sizeof(int) * qbegin.size(), hipMemcpyHostToDevice,
+                             stream),
+              "hipMemcpyAsync(queue)");
         check(hipMemcpyAsync(d_fill, h_fill, sizeof(int) * (nfill + 1), hipMemcpyHostToDevice,
                              stream),
               "hipMemcpyAsync(fill)");
@@ -770,9 +835,18 @@ struct mhd_hip_batch {
 
     BatchTables tables(int g, int st) const
     {
-        return BatchTables{d_insts, d_ctrl, d_tiles,
-                           d_first + static_cast<size_t>(2 * g + st - 1) * (max_waves + 1),
-                           d_fill, d_fill + 1, dry_run};
+        const size_t k = static_cast<size_t>(2 * g + st - 1);
+        const size_t other = static_cast<size_t>(2 * g + (st == 1 ? 1 : 0));
+        return BatchTables{d_insts,
+                           d_ctrl,
+                           d_tiles,
+                           d_first + k * (max_waves + 1),
+                           d_fill,
+                           d_fill + 1,
+                           dry_run,
+                           queued ? d_queue + k * kQueueSlots : nullptr,
+                           d_next + k * mhdhip::kPlanXcds,
+                           d_next + other * mhdhip::kPlanXcds};
     }
 
     // One RK stage of the stepping set: the ghost and wall fill of the masked instances
@@ -795,10 +869,15 @@ struct mhd_hip_batch {
         }
     }
 
-    unsigned launch_key() const { return used_groups | (fill_needed ? 1u << kGroups : 0u); }
+    unsigned launch_key() const
+    {
+        return used_groups | (fill_needed ? 1u << kGroups : 0u) |
+               (queued ? 1u << (kGroups + 1) : 0u);
+    }
 
-    // The captured graph of kSteps steps; captured again when the kernel variants in use (or
-    // the need of the fill) change, since the graph holds one launch per variant.
+    // The captured graph of kSteps steps; captured again when the kernel variants in use, the
+    // need of the fill or the kind of hand-out change, since the graph holds one launch per
+    // variant with the hand-out's table pointers.
     template <typename R, int FAST>
     void ensure_graph()
     {
@@ -846,9 +925,10 @@ struct mhd_hip_batch {
                 launches.push_back(mhdhip::PlanLaunch{&sets[g], group_waves(g, st)});
             }
         }
-        return mhdhip::shortlist_rows(launches,
-                                      std::vector<int>(std::begin(kModelRowCandidates),
-                                                       std::end(kModelRowCandidates)));
+        return mhdhip::shortlist_rows_for(launches,
+                                          std::vector<int>(std::begin(kModelRowCandidates),
+                                                           std::end(kModelRowCandidates)),
+                                          mhdhip::HandOut::kQueued);
     }
 
     template <typename R, int FAST>
@@ -868,12 +948,16 @@ struct mhd_hip_batch {
                 for (int j = 0; j < n; ++j) {
                     const int i = sweep ? n - 1 - j : j;
                     build_tables(modes[i]);
+                    open_launches();
                     launch_stage<R, FAST>(1, 0);
                     launch_stage<R, FAST>(2, 0);
+                    counters_known = true;
                     for (int rep = 0; rep < 3; ++rep) {
                         check(hipEventRecord(tev[0], stream), "hipEventRecord");
+                        open_launches();
                         launch_stage<R, FAST>(1, 0);
                         launch_stage<R, FAST>(2, 0);
+                        counters_known = true;
                         check(hipEventRecord(tev[1], stream), "hipEventRecord");
                         check(hipEventSynchronize(tev[1]), "hipEventSynchronize");
                         float ms = 0.0f;
@@ -1038,7 +1122,9 @@ struct mhd_hip_batch {
                 timed_replay<R, FAST>();
             } else {
                 ensure_graph<R, FAST>();
+                open_launches();
                 check(hipGraphLaunch(graph, stream), "hipGraphLaunch");
+                counters_known = true;
             }
         }
         check(hipEventRecord(ev1, stream), "hipEventRecord");

@@ -32,10 +32,16 @@ struct BatchTables {
     const BatchInst* insts;
     Ctrl* ctrl;  // one control block per instance slot
     const BatchTile* tiles;
-    const int* first;       // this launch's per-wave offsets into tiles (waves + 1)
+    const int* first;       // static hand-out: this launch's per-wave offsets into tiles
     const int* fill_count;  // how many slots fill_list holds
     const int* fill_list;   // masked slots, whose ghosts and solid cells are filled per stage
     int dry;  // timing pass: stage 2 writes U* instead of U and no control field is touched
+    // queued hand-out (null qbegin: the static one): this launch's queue offsets into tiles
+    // (launch_queues(grid) + 1), its queue counters, and the counters of the same variant's
+    // other stage, which this launch zeroes as it starts
+    const int* qbegin;
+    int* next;
+    int* other_next;
 };
 
 // Waves per SIMD each batched stage kernel is compiled for: what its registers allow (fp32
@@ -59,14 +65,66 @@ void k_batch_stage(BatchTables tb, int parity)
     static_assert(FAST == MHD_FAST_FP64, "stepping kernel built in the wrong translation unit");
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = wave_in_block();
-    const int gw = xcd_logical_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
-    const int t_begin = tb.first[gw];
-    const int n_tiles = tb.first[gw + 1] - t_begin;
+    const bool queued = tb.qbegin != nullptr;
+    // static hand-out: this wave's list tiles[t_begin .. t_begin + n_tiles)
+    int t_begin = 0, n_tiles = 0;
+    // queued hand-out: this wave's queue q (its XCD's) of qn tiles from tiles[qb], which wq
+    // waves take, this one first the tile at its index lw among them, then tickets
+    int q = 0, qb = 0, qn = 0, wq = 0, lw = 0;
+    if (queued) {
+        const int nb = static_cast<int>(gridDim.x);
+        const int b = static_cast<int>(blockIdx.x);
+        q = b % launch_queues(nb);
+        qb = uniform_i32(tb.qbegin[q]);
+        qn = uniform_i32(tb.qbegin[q + 1]) - qb;
+        wq = queue_waves(nb, q, kWavesPerBlock);
+        lw = queue_wave_index(nb, b, wave, kWavesPerBlock);
+        if (b == 0 && wave == 0 && lane < kPlanXcds) {
+            atomicExch(&tb.other_next[lane], 0);
+        }
+    } else {
+        const int gw = xcd_logical_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
+        t_begin = tb.first[gw];
+        n_tiles = tb.first[gw + 1] - t_begin;
+    }
     __shared__ R ring[kWavesPerBlock][3][kNC][kWave];
-    // stage 2 takes the wave's tiles in reverse order (and marches each upward), so that it
-    // starts on what stage 1 touched last, and the next stage 1 on what stage 2 touched last
-    for (int j = 0; j < n_tiles; ++j) {
-        const int t = STAGE == 2 ? t_begin + n_tiles - 1 - j : t_begin + j;
+    // Stage 2: the wave holds the largest signal speed of its consecutive tiles of one
+    // instance (as the bits of a non-negative double, the order atomic_max_speed compares)
+    // and adds it to the instance's control block when the instance changes or the list
+    // ends. One atomic per run of tiles instead of one per tile: the atomics of all the tiles
+    // of a large grid meet at one address and are served one after another, and each wave's
+    // next tile waited for its last one.
+    int held = -1;  // instance whose speed the wave holds, -1 for none
+    unsigned long long held_bits = 0ull;
+    auto release = [&]() {
+        if (STAGE == 2 && held >= 0 && lane == 0) {
+            atomicMax(&tb.ctrl[held].chmax[parity ^ 1], held_bits);
+        }
+    };
+    // stage 2 takes the wave's tiles (the queue's) in reverse order (and marches each upward),
+    // so that it starts on what stage 1 touched last, and the next stage 1 on what stage 2
+    // touched last
+    for (int j = 0;; ++j) {
+        int t;
+        if (queued) {
+            int u = lw;
+            if (j > 0) {
+                int v = 0;
+                if (lane == 0) {
+                    v = atomicAdd(&tb.next[q], 1);
+                }
+                u = wq + uniform_i32(v);
+            }
+            if (u >= qn) {
+                break;
+            }
+            t = qb + (STAGE == 2 ? qn - 1 - u : u);
+        } else {
+            if (j >= n_tiles) {
+                break;
+            }
+            t = STAGE == 2 ? t_begin + n_tiles - 1 - j : t_begin + j;
+        }
         const BatchTile tile = tb.tiles[t];
         const BatchInst& in = tb.insts[tile.inst];
         if (!in.enabled) {
@@ -106,12 +164,19 @@ void k_batch_stage(BatchTables tb, int parity)
             src, u0, dst, REMAP ? nullptr : in.mask, g, in.ph, st, strip, r0, r1, dB, r1b,
             ring[wave], lane);
         if (STAGE == 2 && !tb.dry) {
-            const R m = wave_max(local);
-            if (lane == 0) {
-                atomic_max_speed(&ctrl->chmax[parity ^ 1], static_cast<double>(m));
+            const int inst = uniform_i32(tile.inst);
+            if (inst != held) {
+                release();
+                held = inst;
+                held_bits = 0ull;
             }
+            const double m = static_cast<double>(wave_max(local));
+            const unsigned long long bits =
+                static_cast<unsigned long long>(__double_as_longlong(m));
+            held_bits = bits > held_bits ? bits : held_bits;
         }
     }
+    release();
 }
 
 // Ghost fill, then solid-wall fill, of the stage input of every listed instance whose step

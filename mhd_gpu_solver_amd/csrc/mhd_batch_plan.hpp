@@ -1,7 +1,8 @@
 // Host-side planning of the batched engine, free of HIP so that it compiles and is tested on
 // the CPU as well:
-//   * the work partition: which tiles (instance, strip, row range) each wave of the
-//     persistent stage kernel marches, as per-wave lists (CSR offsets into a tile array);
+//   * the work partition: the tiles (instance, strip, row range) of the persistent stage
+//     kernel, either as per-wave lists (CSR offsets into a tile array) or as one queue per
+//     XCD from which the waves take tiles as they finish (plan_queues);
 //   * the accounting of a replay of timed steps: how many of its launches stepped anything;
 //   * the progress check that turns a replay that advanced no stepping instance into an error.
 #pragma once
@@ -278,6 +279,155 @@ inline WavePlan plan_fill(const std::vector<PlanInst>& insts, int waves, int min
     pour(lo, &p.tiles, &p.first, &p.max_cost);
     for (const BatchTile& t : p.tiles) p.total_cost += tile_cost(t.r1 - t.r0);
     return p;
+}
+
+// ---- dynamic hand-out: one tile queue per XCD ----------------------------------------------
+// The stage kernel's waves on XCD x take the tiles of queue x: wave lw of the W_x waves there
+// first takes tile lw, then tickets W_x, W_x + 1, ... from the queue's counter (an atomic add),
+// until a ticket is past the queue's end. A wave that finishes early takes the next tile, so
+// the waves end together without a plan of who takes what; the tiles running at the same time
+// on one XCD are neighbours in the list, so their shared halo rows and strip edges meet in one
+// L2. Each stage's kernel zeroes the other stage's counters of its variant when it starts
+// (the stages alternate on one stream), so no wave has to know that it is the last one.
+constexpr int kPlanXcds = 8;
+
+// Queues of a launch of `blocks` workgroups: one per XCD, or one per workgroup when there are
+// fewer workgroups than XCDs (every queue needs a wave). Workgroup b takes queue b mod
+// queues, which is its XCD (workgroups are dispatched round-robin over the XCDs).
+constexpr int launch_queues(int blocks)
+{
+    return blocks < kPlanXcds ? (blocks < 1 ? 1 : blocks) : kPlanXcds;
+}
+// Waves of `wpb` per workgroup that take queue q, and the index of wave `wave` of workgroup
+// `block` among the waves of its queue. (constexpr: the stage kernel uses them too.)
+constexpr int queue_waves(int blocks, int q, int wpb)
+{
+    return q < blocks ? ((blocks - 1 - q) / launch_queues(blocks) + 1) * wpb : 0;
+}
+constexpr int queue_wave_index(int blocks, int block, int wave, int wpb)
+{
+    return (block / launch_queues(blocks)) * wpb + wave;
+}
+static_assert(queue_waves(1024, 0, 4) == 512 && queue_waves(1025, 0, 4) == 516 &&
+                  queue_waves(1025, 1, 4) == 512 && queue_waves(3, 2, 4) == 4 &&
+                  launch_queues(3) == 3 && queue_wave_index(3, 2, 1, 4) == 1,
+              "waves per queue");
+
+struct QueuePlan {
+    std::vector<BatchTile> tiles;  // queue x holds tiles[begin[x] .. begin[x + 1])
+    std::vector<int> begin;        // queues + 1 offsets
+    long long total_cost = 0;
+    long long max_queue_cost = 0;  // the costliest queue (half-rows)
+};
+
+// Fixed segments of `rows` rows, queued largest instance first (the short tiles of the small
+// grids come last, where the hand-out ends), each instance band by band of `rows` rows, a
+// band's strips left to right (its half-wave tile last); the list is cut into `queues`
+// contiguous queues of about equal load-model cost (launch_queues of the launch's grid).
+inline QueuePlan plan_queues(std::vector<PlanInst> insts, int rows, int queues = kPlanXcds)
+{
+    const int xcds = std::max(1, queues);
+    rows = std::max(1, rows);
+    std::stable_sort(insts.begin(), insts.end(), [](const PlanInst& a, const PlanInst& b) {
+        return static_cast<long long>(a.nx) * a.ny > static_cast<long long>(b.nx) * b.ny;
+    });
+    QueuePlan p;
+    for (const PlanInst& in : insts) {
+        const StripShape s = strip_shape(in.nx, in.ny);
+        for (int a0 = 0; a0 < in.ny; a0 += rows) {
+            for (int st = 0; st < s.wide; ++st) {
+                p.tiles.push_back(make_tile(in.slot, in.nx, in.ny, st, a0,
+                                            std::min(a0 + rows, in.ny)));
+            }
+            if (a0 < s.half_rows) {
+                p.tiles.push_back(make_tile(in.slot, in.nx, in.ny, s.wide, a0,
+                                            std::min(a0 + rows, s.half_rows)));
+            }
+        }
+    }
+    std::vector<long long> prefix(p.tiles.size() + 1, 0);
+    for (size_t k = 0; k < p.tiles.size(); ++k) {
+        prefix[k + 1] = prefix[k] + tile_cost(p.tiles[k].r1 - p.tiles[k].r0);
+    }
+    p.total_cost = prefix.back();
+    p.begin.assign(static_cast<size_t>(xcds) + 1, 0);
+    size_t k = 0;
+    for (int x = 1; x < xcds; ++x) {
+        // the first cut at or past x / xcds of the cost, or the one before it when nearer
+        const long long target = p.total_cost * x / xcds;
+        while (k < p.tiles.size() && prefix[k] < target) ++k;
+        size_t cut = k;
+        if (cut > 0 && target - prefix[cut - 1] < prefix[cut] - target) --cut;
+        cut = std::max(cut, static_cast<size_t>(p.begin[static_cast<size_t>(x) - 1]));
+        p.begin[static_cast<size_t>(x)] = static_cast<int>(cut);
+    }
+    p.begin[static_cast<size_t>(xcds)] = static_cast<int>(p.tiles.size());
+    for (int x = 0; x < xcds; ++x) {
+        p.max_queue_cost = std::max(p.max_queue_cost,
+                                    prefix[static_cast<size_t>(p.begin[x + 1])] -
+                                        prefix[static_cast<size_t>(p.begin[x])]);
+    }
+    return p;
+}
+
+// Load model of the queued hand-out on `waves` waves: the costliest queue spread over its
+// XCD's waves, plus half a tile for the end, where the last waves finish their last tiles at
+// staggered times.
+inline long long queue_model_cost(const std::vector<PlanInst>& insts, int rows, int waves,
+                                  int wpb = 4)
+{
+    waves = std::max(1, waves);
+    const int queues = launch_queues(std::max(1, waves / std::max(1, wpb)));
+    const QueuePlan p = plan_queues(insts, rows, queues);
+    const long long per_xcd = std::max(1, waves / queues);
+    return (p.max_queue_cost + per_xcd - 1) / per_xcd + tile_cost(std::max(1, rows)) / 2;
+}
+
+// The two hand-outs as the load model and the timed choice see them.
+enum class HandOut { kStatic, kQueued };
+
+inline long long handout_cost(const std::vector<PlanInst>& insts, int rows, int waves,
+                              HandOut how)
+{
+    return how == HandOut::kQueued ? queue_model_cost(insts, rows, waves)
+                                   : fixed_max_cost(insts, rows, waves);
+}
+
+// model_rows and shortlist_rows for either hand-out (the static one: the busiest wave of the
+// round-robin plan; the queued one: queue_model_cost).
+inline int model_rows_for(const std::vector<PlanInst>& insts, int waves,
+                          const std::vector<int>& candidates, HandOut how)
+{
+    int best = candidates.empty() ? 32 : candidates.front();
+    long long best_cost = std::numeric_limits<long long>::max();
+    for (const int rows : candidates) {
+        const long long c = handout_cost(insts, rows, waves, how);
+        if (c < best_cost || (c == best_cost && rows > best)) {
+            best_cost = c;
+            best = rows;
+        }
+    }
+    return best;
+}
+
+inline std::vector<int> shortlist_rows_for(const std::vector<PlanLaunch>& launches,
+                                           const std::vector<int>& candidates, HandOut how,
+                                           int count = 3, int pct = 12)
+{
+    std::vector<std::pair<long long, int>> cost;
+    for (const int rows : candidates) {
+        long long c = 0;
+        for (const PlanLaunch& l : launches) {
+            if (!l.insts->empty()) c += handout_cost(*l.insts, rows, l.waves, how);
+        }
+        cost.emplace_back(c, -rows);
+    }
+    std::sort(cost.begin(), cost.end());
+    std::vector<int> out;
+    for (size_t i = 0; i < cost.size() && static_cast<int>(out.size()) < count; ++i) {
+        if (cost[i].first * 100 <= cost.front().first * (100 + pct)) out.push_back(-cost[i].second);
+    }
+    return out;
 }
 
 // ---- replay accounting ---------------------------------------------------------------------
