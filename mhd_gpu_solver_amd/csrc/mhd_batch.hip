@@ -594,6 +594,13 @@ struct mhd_hip_batch {
         if (scheme.limiter != MHD_HIP_MINMOD && scheme.limiter != MHD_HIP_MC) {
             throw InvalidArgument("unknown slope limiter");
         }
+        // the stage kernels address a lane's cell by a 32-bit byte offset within its plane
+        if (static_cast<unsigned long long>(d.nx + 4) * static_cast<unsigned long long>(d.ny + 4) *
+                elem >=
+            (1ull << 32)) {
+            throw InvalidArgument("instance grid too large for the batch engine (a component "
+                                  "plane of 4 GiB or more); use the single-run solver");
+        }
         const size_t cells = static_cast<size_t>(d.nx) * d.ny;
         if (mask != nullptr && mask_len != cells) {
             throw InvalidArgument("FluidMask size does not match Grid2D interior");

@@ -686,6 +686,36 @@ struct Planes {
     }
 };
 
+// A copy of x in a register of its own (one register move the compiler cannot elide).
+__device__ __forceinline__ float own_register(float x)
+{
+    float y;
+    asm("v_mov_b32 %0, %1" : "=v"(y) : "v"(x));
+    return y;
+}
+__device__ __forceinline__ double own_register(double x)
+{
+    double y;
+    asm("v_mov_b64 %0, %1" : "=v"(y) : "v"(x));
+    return y;
+}
+
+// MUSCL face state w + SIGN d / 2 (include/mhd/kernels/muscl_hll.hpp:59-68). The exact fp64
+// build rounds the product and the sum separately, as the CPU path does; the FP32 and the
+// contracted fp64 builds take one fused multiply-add per side (the compiler does not fuse the
+// product that both sides share by itself).
+template <int SIGN, typename R>
+__device__ __forceinline__ R face_state(R w, R d)
+{
+    if constexpr (sizeof(R) == 4) {
+        return __builtin_fmaf(SIGN * 0.5f, d, w);
+    } else if constexpr (MHD_FAST_FP64) {
+        return __builtin_fma(SIGN * 0.5, d, w);
+    } else {
+        return SIGN > 0 ? w + R(0.5) * d : w - R(0.5) * d;
+    }
+}
+
 // Stage 1 marches its rows in increasing order and stage 2 in decreasing order, so that each
 // stage starts on the rows the previous one touched last (still in the memory-side cache).
 template <int STAGE>
@@ -811,6 +841,16 @@ __device__ __forceinline__ R march_tile(const Planes& src, const Planes& u0, con
         const bool update = DIR > 0 ? c >= r0 + 1 : c <= r1 - 2;
         ring_put(c + DIR, Uraw);
         to_prim(Uraw, gm1, Wn);  // row c + DIR
+        // The primitives that equal their conserved values (rho, B, psi) take registers of
+        // their own, so that the raw row's registers are free once the row is in the ring and
+        // the next row can be loaded into them: otherwise the loaded values are moved into
+        // place at the end of the step, and each move waits for its load and for everything
+        // issued before it, the update's stores included.
+        Wn[RHO] = own_register(Wn[RHO]);
+        Wn[BX] = own_register(Wn[BX]);
+        Wn[BY] = own_register(Wn[BY]);
+        Wn[BZ] = own_register(Wn[BZ]);
+        Wn[PSI] = own_register(Wn[PSI]);
         // stage 2 updates row c - DIR from U at the start of the step: with EARLY those loads
         // are issued now, so that their latency overlaps the face computations (column clamped
         // like the row loads; a step with no update loads a row of the tile that it does not
@@ -835,8 +875,8 @@ __device__ __forceinline__ R march_tile(const Planes& src, const Planes& u0, con
 #pragma unroll
         for (int k = 0; k < kNC; ++k) {
             const R d = limit_slope<LIM>(Wb[k] - Wlo[k], Whi[k] - Wb[k]);
-            const R lo = Wb[k] - half * d;  // the state at face c - 1/2
-            const R hi = Wb[k] + half * d;  // the state at face c + 1/2
+            const R lo = face_state<-1>(Wb[k], d);  // the state at face c - 1/2
+            const R hi = face_state<1>(Wb[k], d);   // the state at face c + 1/2
             Wf[k] = DIR > 0 ? lo : hi;
             Wn_c[k] = DIR > 0 ? hi : lo;
         }
@@ -885,8 +925,8 @@ __device__ __forceinline__ R march_tile(const Planes& src, const Planes& u0, con
                     Wl[k] = from_left(Wa[k]);
                     const R wr = from_right(Wa[k]);
                     const R d = limit_slope<LIM>(Wa[k] - Wl[k], wr - Wa[k]);
-                    WRx[k] = Wa[k] - half * d;
-                    WLx[k] = from_left(Wa[k] + half * d);
+                    WRx[k] = face_state<-1>(Wa[k], d);
+                    WLx[k] = from_left(face_state<1>(Wa[k], d));
                 }
                 R F[kNC];
                 hll_face<0>(WLx, WRx, Wl, Wa, gamma, inv_gm1, c_h, F, x_live);
