@@ -333,17 +333,16 @@ int mhd_hip_batch_totals(mhd_hip_batch* b, int id, double sums[9]);
 int mhd_hip_batch_rel_l1(mhd_hip_batch* a, int id_a, mhd_hip_batch* ref, int id_ref,
                          double l1[9]);
 
-/* Work partition of the batch kernel. Each wave of the persistent grid marches its own list
- * of tiles (instance, strip, row range). 0 (default): fixed segments whose length a load
- * model picks whenever the stepping set changes (it replays the round-robin hand-out of the
- * tiles to the waves for candidate lengths and keeps the one whose busiest wave has the least
- * work; suits a set that changes, as in serving); -1: before the first replay, up to three
- * lengths the load model ranks within 12 % of its best are timed on dry steps (stage 2 writes
- * the scratch state, so no instance advances) and the fastest is kept (suits a set that stays
- * the same); > 0: fixed segments of that many rows handed out
- * round-robin; < -1: the capacity fill, every strip cut into row pieces of at least -rows rows
- * poured into the waves in order up to a common capacity. Results do not depend on this
- * value. */
+/* Work partition of the batch kernel: tiles (instance, strip, row range) of fixed row
+ * segments, in one queue per XCD from which the waves of the persistent grid take tiles as
+ * they finish the last (largest instances first, band by band). 0 (default): the segment
+ * length a load model picks whenever the stepping set changes (the costliest queue spread
+ * over its XCD's waves plus half a tile; suits a set that changes, as in serving); -1: before
+ * the first replay, up to three lengths the load model ranks within 12 % of its best are
+ * timed on dry steps (stage 2 writes the scratch state, so no instance advances) and the
+ * fastest is kept (suits a set that stays the same); > 0: segments of that many rows;
+ * < -1: the capacity fill, every strip cut into row pieces of at least -rows rows poured
+ * into per-wave lists in order up to a common capacity. Results do not depend on this value. */
 int mhd_hip_batch_set_seg_rows(mhd_hip_batch* b, int rows);
 
 int mhd_hip_batch_info(const mhd_hip_batch* b, mhd_hip_batch_stats* stats);
@@ -352,6 +351,15 @@ int mhd_hip_batch_info(const mhd_hip_batch* b, mhd_hip_batch_stats* stats);
  * eagerly with a HIP event pair around every stage kernel; totals per stage (ms, launches). */
 int mhd_hip_batch_set_kernel_timing(mhd_hip_batch* b, int enable);
 int mhd_hip_batch_kernel_times(const mhd_hip_batch* b, double total_ms[2], long long launches[2]);
+
+/* Measured bandwidth of one RK stage's memory traffic without its arithmetic, on the current
+ * device: a streaming kernel that reads 9 planes and writes 9 (stage 1) or reads 18 and writes
+ * 9 in place (stage 2), `cells` elements per plane in fp64 or FP32 (precision), timed with HIP
+ * events over `reps` launches after two untimed ones; *gbs = bytes moved / mean launch time.
+ * Allocates and frees its own buffers (2 x 9 x cells elements). The bandwidth the memory
+ * system delivers for the stage's mix of reads and writes, reported beside the published HBM
+ * peak. No counterpart in the reference (a measurement aid). */
+int mhd_hip_stream_bandwidth(int precision, long long cells, int stage, int reps, double* gbs);
 
 #ifdef __cplusplus
 }

@@ -20,11 +20,12 @@ from .solver import (  # noqa: F401
     hip_available,
     init_ic,
     metal_available,
+    stream_bandwidth,
     wall_plan,
 )
 
 __all__ = [
     "BatchEngine", "CapacityFull", "instance_bytes", "FIELDS", "BoundaryConditionType", "DeviceSolver", "MHDSolver", "NonIdealConfig",
     "SlopeLimiter", "SolveParams", "SolveResult", "available_backends", "cuda_available",
-    "hip_available", "init_ic", "metal_available", "wall_plan", "state_io",
+    "hip_available", "init_ic", "metal_available", "stream_bandwidth", "wall_plan", "state_io",
 ]

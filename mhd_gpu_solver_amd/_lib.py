@@ -104,6 +104,8 @@ SIGNATURES = [
     ("mhd_hip_batch_info", C.c_int, [C.c_void_p, C.POINTER(BatchStats)]),
     ("mhd_hip_batch_max_div_b", C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double)]),
     ("mhd_hip_max_div_b", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    ("mhd_hip_stream_bandwidth", C.c_int, [C.c_int, C.c_longlong, C.c_int, C.c_int,
+                                           C.POINTER(C.c_double)]),
     ("mhd_hip_rel_l1", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]),
     ("mhd_hip_batch_rel_l1", C.c_int,
      [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_double)]),

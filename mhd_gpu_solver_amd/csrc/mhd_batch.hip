@@ -186,6 +186,8 @@ struct mhd_hip_batch {
     static constexpr int kQueueSlots = mhdhip::kPlanXcds + 1;
     int* d_queue = nullptr;
     int* d_next = nullptr;
+    int* d_live = nullptr;  // instances of the tile lists still to step, per step parity
+                            // (BatchTables::live)
     size_t tile_cap = 0;
 
     BatchInst* h_insts = nullptr;  // pinned staging
@@ -194,6 +196,8 @@ struct mhd_hip_batch {
     int* h_first = nullptr;
     int* h_fill = nullptr;
     int* h_queue = nullptr;
+    int* h_live = nullptr;
+    std::vector<char> tabled;  // slots whose tiles the current tables hold
     size_t h_tile_cap = 0;
     double* h_stage = nullptr;  // pinned staging for state upload/download
     size_t h_stage_bytes = 0;
@@ -248,12 +252,14 @@ struct mhd_hip_batch {
         if (d_fill) (void)hipFree(d_fill);
         if (d_queue) (void)hipFree(d_queue);
         if (d_next) (void)hipFree(d_next);
+        if (d_live) (void)hipFree(d_live);
         if (h_insts) (void)hipHostFree(h_insts);
         if (h_ctrl) (void)hipHostFree(h_ctrl);
         if (h_tiles) (void)hipHostFree(h_tiles);
         if (h_first) (void)hipHostFree(h_first);
         if (h_fill) (void)hipHostFree(h_fill);
         if (h_queue) (void)hipHostFree(h_queue);
+        if (h_live) (void)hipHostFree(h_live);
         if (h_stage) (void)hipHostFree(h_stage);
         if (totals_partial) (void)hipFree(totals_partial);
         if (totals_out) (void)hipFree(totals_out);
@@ -388,6 +394,12 @@ struct mhd_hip_batch {
                              stream),
               "hipMemcpyAsync");
         check(hipMemsetAsync(d_next, 0, sizeof(int) * nnext, stream), "hipMemsetAsync(next)");
+        check(hipMalloc(reinterpret_cast<void**>(&d_live), 2 * sizeof(int)), "hipMalloc(live)");
+        check(hipHostMalloc(reinterpret_cast<void**>(&h_live), 2 * sizeof(int), 0),
+              "hipHostMalloc");
+        h_live[0] = h_live[1] = 0;
+        check(hipMemsetAsync(d_live, 0, 2 * sizeof(int), stream), "hipMemsetAsync(live)");
+        tabled.assign(static_cast<size_t>(max_inst), 0);
         check(hipHostMalloc(reinterpret_cast<void**>(&h_insts), sizeof(BatchInst) * max_inst, 0),
               "hipHostMalloc");
         check(hipHostMalloc(reinterpret_cast<void**>(&h_ctrl), sizeof(Ctrl) * max_inst, 0),
@@ -724,10 +736,23 @@ struct mhd_hip_batch {
                                       mhdhip::HandOut::kQueued);
     }
 
-    // Before a sequence of stage launches: zero the queue counters if an earlier sequence was
-    // cut short (counters_known is set again once the sequence is enqueued whole).
+    // Before a sequence of stage launches: the count of tabled instances still to step (from
+    // the control blocks as last read; the stream has drained since, so the pinned word is
+    // free), and the queue counters zeroed if an earlier sequence was cut short
+    // (counters_known is set again once the sequence is enqueued whole).
     void open_launches()
     {
+        int live = 0;
+        for (int si = 0; si < max_instances; ++si) {
+            const Slot& s = slots[si];
+            live += (tabled[si] && s.used && h_ctrl[si].t[0] < s.desc.t_end &&
+                     h_ctrl[si].steps[0] < s.desc.max_steps)
+                        ? 1
+                        : 0;
+        }
+        h_live[0] = h_live[1] = live;  // every sequence starts at parity 0
+        check(hipMemcpyAsync(d_live, h_live, 2 * sizeof(int), hipMemcpyHostToDevice, stream),
+              "hipMemcpyAsync(live)");
         if (!counters_known) {
             check(hipMemsetAsync(d_next, 0,
                                  sizeof(int) * static_cast<size_t>(kGroups) * 2 *
@@ -758,8 +783,10 @@ struct mhd_hip_batch {
         used_groups = 0;
         last_rows = 0;
         queued = false;
+        std::fill(tabled.begin(), tabled.end(), 0);
         for (int g = 0; g < kGroups; ++g) {
             const std::vector<mhdhip::PlanInst> insts = stepping_set(g);
+            for (const mhdhip::PlanInst& in : insts) tabled[static_cast<size_t>(in.slot)] = 1;
             for (int st = 1; st <= 2; ++st) {
                 const int base = static_cast<int>(tiles.size());
                 int* f = first.data() + static_cast<size_t>(2 * g + st - 1) * (max_waves + 1);
@@ -811,6 +838,7 @@ struct mhd_hip_batch {
             in.g = geom_of(s.desc);
             in.ph = physics_of(s);
             in.enabled = 1;
+            in.stepping = tabled[static_cast<size_t>(si)];
             if (s.masked && !s.finished && !s.paused) {
                 h_fill[1 + nfill++] = si;
             }
@@ -853,7 +881,9 @@ struct mhd_hip_batch {
                            dry_run,
                            queued ? d_queue + k * kQueueSlots : nullptr,
                            d_next + k * mhdhip::kPlanXcds,
-                           d_next + other * mhdhip::kPlanXcds};
+                           d_next + other * mhdhip::kPlanXcds,
+                           d_live,
+                           max_instances};
     }
 
     // One RK stage of the stepping set: the ghost and wall fill of the masked instances

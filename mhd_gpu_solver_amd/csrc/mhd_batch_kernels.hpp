@@ -26,6 +26,7 @@ struct BatchInst {
     Physics ph;
     int plan_count;
     int enabled;
+    int stepping;  // its tiles are in the current tile lists
 };
 
 struct BatchTables {
@@ -42,6 +43,13 @@ struct BatchTables {
     const int* qbegin;
     int* next;
     int* other_next;
+    // Instances of the tile lists still to step, per step parity: live[p] is the count for
+    // the step with parity p, constant while its launches run. Stage 1 copies it to
+    // live[p ^ 1], stage 2 takes an instance off live[p ^ 1] when it finishes; the host sets
+    // live[0] before each launch sequence. A launch with none returns at once (after the
+    // bookkeeping an inactive step does) instead of visiting every tile of finished instances.
+    int* live;
+    int slots;  // instance slots (entries of insts and ctrl)
 };
 
 // Waves per SIMD each batched stage kernel is compiled for: what its registers allow (fp32
@@ -82,7 +90,34 @@ void k_batch_stage(BatchTables tb, int parity)
         if (b == 0 && wave == 0 && lane < kPlanXcds) {
             atomicExch(&tb.other_next[lane], 0);
         }
-    } else {
+    }
+    const int live = uniform_i32(tb.live[parity]);
+    if (STAGE == 1 && !tb.dry && blockIdx.x == 0 && threadIdx.x == 0) {
+        tb.live[parity ^ 1] = live;
+    }
+    if (live <= 0) {
+        // no instance of the lists steps: the step bookkeeping of each of them, as its lead
+        // tile does for an inactive step (none is pending, so none has dt <= 0 to report)
+        if (!tb.dry && blockIdx.x == 0) {
+            for (int si = static_cast<int>(threadIdx.x); si < tb.slots; si += kBlock) {
+                const BatchInst& in = tb.insts[si];
+                if (!in.enabled || !in.stepping) {
+                    continue;
+                }
+                Ctrl* ctrl = tb.ctrl + si;
+                const int q = parity ^ 1;
+                if (STAGE == 1) {
+                    ctrl->chmax[q] = 0ull;
+                } else {
+                    ctrl->t[q] = ctrl->t[parity];
+                    ctrl->steps[q] = ctrl->steps[parity];
+                    ctrl->ch_last[q] = ctrl->ch_last[parity];
+                }
+            }
+        }
+        return;
+    }
+    if (!queued) {
         const int gw = xcd_logical_block(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
         t_begin = tb.first[gw];
         n_tiles = tb.first[gw + 1] - t_begin;
@@ -142,9 +177,14 @@ void k_batch_stage(BatchTables tb, int parity)
         }
         if (STAGE == 2 && lead && !tb.dry) {
             const int q = parity ^ 1;
-            ctrl->t[q] = st.active ? st.t + st.dt : st.t;
-            ctrl->steps[q] = st.active ? st.steps + 1 : st.steps;
+            const double t_new = st.active ? st.t + st.dt : st.t;
+            const int steps_new = st.active ? st.steps + 1 : st.steps;
+            ctrl->t[q] = t_new;
+            ctrl->steps[q] = steps_new;
             ctrl->ch_last[q] = st.active ? st.c_h : ctrl->ch_last[parity];
+            if (st.active && !(t_new < in.ph.t_end && steps_new < in.ph.max_steps)) {
+                atomicSub(&tb.live[q], 1);  // this instance has finished
+            }
         }
         if (!st.active) {
             continue;

@@ -130,6 +130,19 @@ def hip_available() -> bool:
     return bool(_lib.lib().mhd_hip_available())
 
 
+def stream_bandwidth(precision: str, cells: int, stage: int, reps: int = 20) -> float:
+    """GB/s of a streaming kernel with one RK stage's memory traffic and none of its arithmetic
+    (stage 1: 9 planes read, 9 written; stage 2: 18 read, 9 written in place; `cells` elements
+    per plane), measured on the current device (include/mhd_hip.h, mhd_hip_stream_bandwidth).
+    What the memory system delivers for the stage's mix, beside the published HBM peak."""
+    if precision not in _lib.PRECISIONS:
+        raise ValueError("precision must be 'fp64', 'fp32' or 'fp64_fast'")
+    v = C.c_double()
+    _lib.check(_lib.lib().mhd_hip_stream_bandwidth(_lib.PRECISIONS[precision], int(cells),
+                                                   int(stage), int(reps), C.byref(v)))
+    return v.value
+
+
 def cuda_available() -> bool:
     return False
 

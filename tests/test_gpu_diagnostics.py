@@ -151,3 +151,12 @@ def test_batch_rel_l1_across_engines(hip):
     assert np.allclose(got, want, rtol=1e-13, atol=0.0), (got, want)
     assert np.array_equal(engines["fp64"].rel_l1(ids["fp64"], engines["fp64"], ids["fp64"]),
                           np.zeros(9))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_stream_bandwidth_is_a_plausible_rate(hip, precision):
+    """The streaming measurement beside the roofline: positive, and below the published HBM
+    peak of 8 TB/s (plus timer slack) for both stage mixes."""
+    for stage in (1, 2):
+        gbs = hip.stream_bandwidth(precision, 4_000_000, stage, reps=5)
+        assert 100.0 < gbs < 8000.0 * 1.05

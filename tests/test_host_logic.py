@@ -140,3 +140,15 @@ def test_no_cpu_execution_settings():
     for name in ("set_num_threads", "num_threads", "set_simd_enabled", "simd_enabled",
                  "simd_backend", "simd_width"):
         assert not hasattr(m, name), name
+
+
+def test_stream_bandwidth_argument_validation_without_device():
+    """mhd_hip_stream_bandwidth checks its arguments before it touches a device."""
+    lib = _lib.lib()
+    v = C.c_double(-1.0)
+    assert lib.mhd_hip_stream_bandwidth(_lib.FP32, 1000, 3, 5, C.byref(v)) == _lib.EINVAL
+    assert lib.mhd_hip_stream_bandwidth(_lib.FP32, 0, 2, 5, C.byref(v)) == _lib.EINVAL
+    assert lib.mhd_hip_stream_bandwidth(_lib.FP64, 1000, 1, 0, C.byref(v)) == _lib.EINVAL
+    assert lib.mhd_hip_stream_bandwidth(7, 1000, 1, 5, C.byref(v)) == _lib.EINVAL
+    assert lib.mhd_hip_stream_bandwidth(_lib.FP64, 1000, 1, 5, None) == _lib.EINVAL
+    assert v.value == -1.0
