@@ -117,17 +117,27 @@ def measured_traffic(kernel_prefix, elem_bytes, workload, kernel="k_stage", fast
     return None
 
 
-def roofline(kernels, cells, ms_per_step, elem_bytes, name, traffic=None, mask_bytes=0):
+def mix_ceiling(m, precision, cells):
+    """Measured GB/s of a plain streaming kernel with each stage's read/write mix at this many
+    cells per plane (no arithmetic): what the memory system delivers for that mix on this
+    device, reported beside the published peak."""
+    return {st: round(m.stream_bandwidth(precision, cells, st), 1) for st in (1, 2)}
+
+
+def roofline(kernels, cells, ms_per_step, elem_bytes, name, traffic=None, mask_bytes=0,
+             ceiling=None):
     """Dominant stage kernel: algorithmic bytes per launch (stage 1 reads U, writes U*: 18
     values per interior cell; stage 2 reads U*, U, writes U: 27; with a mask, its byte per
-    interior cell in each stage) over its mean duration."""
+    interior cell in each stage) over its mean duration. `ceiling`: mix_ceiling's measurement
+    for the same cells, added as the dominant stage's streaming bandwidth and the fraction of
+    it achieved."""
     per_cell = {0: 18 * elem_bytes + mask_bytes, 1: 27 * elem_bytes + mask_bytes}
     avg = [kernels[i][0] / max(1, kernels[i][1]) for i in (0, 1)]  # ms
     dom = 0 if avg[0] > avg[1] else 1
     bytes_launch = per_cell[dom] * cells
     achieved = bytes_launch / (avg[dom] * 1e-3) / 1e9
     step_gbs = (45 * elem_bytes + 2 * mask_bytes) * cells / (ms_per_step * 1e-3) / 1e9
-    return {
+    out = {
         "bound": "hbm", "kernel": f"{name}<{dom + 1}>", "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic(dom) if traffic else None,
@@ -135,6 +145,10 @@ def roofline(kernels, cells, ms_per_step, elem_bytes, name, traffic=None, mask_b
         "stage_avg_ms": [round(a, 4) for a in avg],
         "step_achieved_gbs": round(step_gbs, 1), "step_frac": round(step_gbs / HBM_PEAK_GBS, 4),
     }
+    if ceiling:
+        out["stream_mix_gbs"] = ceiling[dom + 1]
+        out["frac_of_stream_mix"] = round(achieved / ceiling[dom + 1], 4)
+    return out
 
 
 # ---- configured single runs (Orszag-Tang 256^2 to t = 0.5; rotor 1024^2 non-ideal + walls) ---
@@ -207,11 +221,20 @@ def _config_summary(m, legs, mask_bytes):
 
 # ---- ensemble -------------------------------------------------------------------------------
 
+# Untimed steps of the timed instances themselves, run just before their timed steps: the
+# admission of the ensemble (host states into the arena) leaves the device idle for tens of
+# milliseconds, and the clocks that drop meanwhile take about two replays to come back, which
+# a run in service never sees. The timed region is still exactly `steps` steps of every
+# instance (its steps LEAD_STEPS + 1 .. LEAD_STEPS + steps).
+LEAD_STEPS = 8
+
+
 def ensemble_batched(m, specs, states, precision, steps, warmup, seg_rows=0):
     """The whole ensemble in one batch engine: `warmup` steps on each launch path (eager with
     events, then the graph: tiling, capture), then `steps` steps timed with events around every
     stage kernel, then the same on the graph path, whose rate is the leg's value. Each timed
-    region starts after the admission and the planning of the admitted set (`prepare`)."""
+    region follows LEAD_STEPS untimed steps of the same instances (after their admission) and
+    the planning of the admitted set (`prepare`)."""
     budget = sum(m.instance_bytes(precision, s.n, s.n) for s in specs) + (64 << 20)
     eng = m.BatchEngine(precision, limiter=m.SlopeLimiter.Minmod, hbm_budget=budget,
                         max_instances=len(specs))
@@ -228,9 +251,10 @@ def ensemble_batched(m, specs, states, precision, steps, warmup, seg_rows=0):
         for i in ids:
             eng.remove(i)
     t0 = time.perf_counter()
-    ids = admit(steps)                               # host fp64 states into the HBM arena
+    ids = admit(LEAD_STEPS + steps)                  # host fp64 states into the HBM arena
     admit_s = time.perf_counter() - t0
     before = {i: eng.totals(i) for i, s in zip(ids, specs) if s.bc == (0, 0)}
+    eng.advance(max_replays=1)                       # LEAD_STEPS untimed steps
     eng.set_kernel_timing(True)
     eng.prepare()                                    # the admitted set's tile tables
     t0 = time.perf_counter()
@@ -247,7 +271,8 @@ def ensemble_batched(m, specs, states, precision, steps, warmup, seg_rows=0):
     results = [eng.download(i) for i in ids]
     for i in ids:
         eng.remove(i)
-    ids = admit(steps)
+    ids = admit(LEAD_STEPS + steps)
+    eng.advance(max_replays=1)                       # LEAD_STEPS untimed steps
     t0 = time.perf_counter()
     eng.prepare()                                    # the admitted set's tables and graph
     prepare_s = time.perf_counter() - t0
@@ -267,7 +292,9 @@ def ensemble_batched(m, specs, states, precision, steps, warmup, seg_rows=0):
 
 def ensemble_plain(m, specs, states, steps):
     """Each instance alone on the plain fp64 single-run solver (the reference's one-solve-per-
-    DeviceSolver model); only the stepping calls are timed (state resident before each)."""
+    DeviceSolver model); only the stepping calls are timed (state resident before each). Each
+    instance runs LEAD_STEPS + `steps` steps, as in the batched legs, so that the results
+    compare; the rate counts all of them."""
     devs = {}
     total = 0.0
     results = []
@@ -282,13 +309,14 @@ def ensemble_plain(m, specs, states, steps):
         dev = devs[key]
         dev.upload(U)
         t0 = time.perf_counter()
-        dev.advance_steps(p, steps)
+        dev.advance_steps(p, LEAD_STEPS + steps)
         total += time.perf_counter() - t0
         out = np.zeros_like(U)
         dev.download(out)
         results.append(out)
     cells = sum(s.n * s.n for s in specs)
-    return dict(mcups=cells * steps / total / 1e6, seconds=total, results=results)
+    return dict(mcups=cells * (LEAD_STEPS + steps) / total / 1e6, seconds=total,
+                results=results)
 
 
 def backend_compare(m, sizes, t_end, precisions=("fp64", "fp32"), repeats=3):
@@ -467,6 +495,7 @@ def main():
             l1 = {f: float(f"{v:.3e}") for f, v in zip(m.FIELDS[:8], worst)}
             # the batched fp64 engine reproduces the single-run fp64 kernel bit for bit
             same = all(np.array_equal(a, b) for a, b in zip(b64["results"], plain["results"]))
+        ceil64 = mix_ceiling(m, "fp64", b64["cells"])
         out.update({
             "value": round(b32["mcups"], 2),
             "ms_per_step": round(b32["ms_per_step"], 4),
@@ -484,7 +513,8 @@ def main():
                                  "k_batch_stage",
                                  traffic=lambda dom: measured_traffic(
                                      f"k_batch_stage<{dom + 1}", 4, f"ensemble_{len(specs)}",
-                                     "k_batch_stage")),
+                                     "k_batch_stage"),
+                                 ceiling=mix_ceiling(m, "fp32", b32["cells"])),
             "seg_rows": b32["seg_rows"], "persistent_blocks": b32["blocks"],
             "row_choice": {-1: "timed", 0: "load model"}.get(args.batch_rows, "fixed"),
             "drift_fp32_periodic_max": b32["drift"],
@@ -499,7 +529,8 @@ def main():
                                       "k_batch_stage",
                                       traffic=lambda dom: measured_traffic(
                                           f"k_batch_stage<{dom + 1}", 8,
-                                          f"ensemble_{len(specs)}", "k_batch_stage", fast=1))},
+                                          f"ensemble_{len(specs)}", "k_batch_stage", fast=1),
+                                      ceiling=ceil64)},
             "batched_fp64": {"value": round(b64["mcups"], 2),
                              "drift_periodic_max": b64["drift"],
                              "ms_per_step": round(b64["ms_per_step"], 4),
@@ -509,7 +540,8 @@ def main():
                                  "k_batch_stage",
                                  traffic=lambda dom: measured_traffic(
                                      f"k_batch_stage<{dom + 1}", 8, f"ensemble_{len(specs)}",
-                                     "k_batch_stage"))},
+                                     "k_batch_stage"),
+                                 ceiling=ceil64)},
         })
         if plain:
             out["plain_fp64"] = {"value": round(plain["mcups"], 2),
@@ -532,6 +564,7 @@ def main():
         log(f"single run {n}x{n}")
         U0 = m.init_ic("orszag_tang_b0", n, n, gamma=5.0 / 3.0, b0=B0_STD)
         legs = {}
+        ceilings = {eb: mix_ceiling(m, "fp64" if eb == 8 else "fp32", n * n) for eb in (8, 4)}
         for prec, eb in (("fp64", 8), ("fp32", 4), ("fp64_fast", 8)):
             leg = single_leg(m, n, prec, args.single_steps, args.warmup, U0)
             log(f"single {prec}: {leg['mcups']:.1f} Mcell-updates/s ({leg['ms_per_step']:.3f} ms/step)")
@@ -543,7 +576,8 @@ def main():
                                      "k_stage",
                                      traffic=lambda dom, eb=eb, prec=prec: measured_traffic(
                                          f"k_stage<{dom + 1}", eb, f"orszag_tang_{n}",
-                                         fast=int(prec == "fp64_fast"))),
+                                         fast=int(prec == "fp64_fast")),
+                                     ceiling=ceilings[eb]),
             }
         out["single_run"] = {"workload": f"orszag_tang_{n}x{n}_single_run", **legs}
         if args.only == "single":
