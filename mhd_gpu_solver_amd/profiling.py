@@ -30,6 +30,8 @@ NOOP_US = 50.0
 
 
 def _short(name: str) -> str:
+    """``void mhdhip::(anonymous namespace)::k_x<float>(float const*, ...)`` -> ``k_x<float>``."""
+    name = name.replace("(anonymous namespace)::", "")
     return name.split("(")[0].replace("void ", "").replace("mhdhip::", "")
 
 
