@@ -730,10 +730,12 @@ struct mhd_hip_batch {
     static int resolve(int mode, const std::vector<mhdhip::PlanInst>& insts, int waves)
     {
         if (mode != 0) return mode;
-        return mhdhip::model_rows_for(insts, waves,
-                                      std::vector<int>(std::begin(kModelRowCandidates),
-                                                       std::end(kModelRowCandidates)),
-                                      mhdhip::HandOut::kQueued);
+        std::vector<int> cands;
+        for (const int r : kModelRowCandidates) {
+            if (r >= mhdhip::kPickMinRows) cands.push_back(r);
+        }
+        return mhdhip::model_rows_for(insts, waves, cands, mhdhip::HandOut::kQueued,
+                                      mhdhip::kPickTileOverhead);
     }
 
     // Before a sequence of stage launches: the count of tabled instances still to step (from

@@ -67,8 +67,9 @@ inline BatchTile make_tile(int slot, int nx, int ny, int st, int a0, int a1)
     return BatchTile{slot, st, ng + a0, ng + a1, -1, 0};
 }
 
-// Cost of a tile of L rows in half-rows: it marches L + 2 rows and loads L + 3.
-inline long long tile_cost(int rows) { return 2ll * rows + 5; }
+// Cost of a tile of L rows in half-rows: it marches L + 2 rows and loads L + 3 (`overhead`
+// half-rows for the three extra row loads and the tile's start-up; see kPickTileOverhead).
+inline long long tile_cost(int rows, int overhead = 5) { return 2ll * rows + overhead; }
 
 // Per-wave tile lists: the tiles of wave w are tiles[first[w] .. first[w + 1]).
 struct WavePlan {
@@ -320,11 +321,27 @@ struct QueuePlan {
     long long max_queue_cost = 0;  // the costliest queue (half-rows)
 };
 
-// Fixed segments of `rows` rows, queued largest instance first (the short tiles of the small
-// grids come last, where the hand-out ends), each instance band by band of `rows` rows, a
-// band's strips left to right (its half-wave tile last); the list is cut into `queues`
-// contiguous queues of about equal load-model cost (launch_queues of the launch's grid).
-inline QueuePlan plan_queues(std::vector<PlanInst> insts, int rows, int queues = kPlanXcds)
+// A strip of n rows cut into segments of at most `rows` rows: ceil(n / rows) segments whose
+// lengths differ by at most one, segment j starting at segment_start(n, count, j). (A cut at
+// every `rows` rows leaves a short last segment, e.g. 128 rows at 24 end in one of 8; a short
+// tile still pays its halo rows and its start-up, and measured much slower per row.)
+constexpr int segment_count(int n, int rows) { return (n + rows - 1) / rows; }
+constexpr int segment_start(int n, int count, int j)
+{
+    return static_cast<int>(static_cast<long long>(n) * j / count);
+}
+static_assert(segment_count(128, 24) == 6 && segment_start(128, 6, 1) == 21 &&
+                  segment_start(128, 6, 6) == 128 && segment_count(64, 16) == 4 &&
+                  segment_start(64, 4, 3) == 48,
+              "even segments");
+
+// Segments of at most `rows` rows, even within each strip, queued largest instance first (the
+// short tiles of the small grids come last, where the hand-out ends), each instance band by
+// band, a band's strips left to right (its half-wave tile last); the list is cut into
+// `queues` contiguous queues of about equal load-model cost (launch_queues of the launch's
+// grid).
+inline QueuePlan plan_queues(std::vector<PlanInst> insts, int rows, int queues = kPlanXcds,
+                             int overhead = 5)
 {
     const int xcds = std::max(1, queues);
     rows = std::max(1, rows);
@@ -334,20 +351,24 @@ inline QueuePlan plan_queues(std::vector<PlanInst> insts, int rows, int queues =
     QueuePlan p;
     for (const PlanInst& in : insts) {
         const StripShape s = strip_shape(in.nx, in.ny);
-        for (int a0 = 0; a0 < in.ny; a0 += rows) {
+        const int kw = segment_count(in.ny, rows);
+        const int kh = s.half ? segment_count(s.half_rows, rows) : 0;  // kh <= kw
+        for (int j = 0; j < kw; ++j) {
+            const int a0 = segment_start(in.ny, kw, j);
+            const int a1 = segment_start(in.ny, kw, j + 1);
             for (int st = 0; st < s.wide; ++st) {
-                p.tiles.push_back(make_tile(in.slot, in.nx, in.ny, st, a0,
-                                            std::min(a0 + rows, in.ny)));
+                p.tiles.push_back(make_tile(in.slot, in.nx, in.ny, st, a0, a1));
             }
-            if (a0 < s.half_rows) {
-                p.tiles.push_back(make_tile(in.slot, in.nx, in.ny, s.wide, a0,
-                                            std::min(a0 + rows, s.half_rows)));
+            if (j < kh) {
+                p.tiles.push_back(make_tile(in.slot, in.nx, in.ny, s.wide,
+                                            segment_start(s.half_rows, kh, j),
+                                            segment_start(s.half_rows, kh, j + 1)));
             }
         }
     }
     std::vector<long long> prefix(p.tiles.size() + 1, 0);
     for (size_t k = 0; k < p.tiles.size(); ++k) {
-        prefix[k + 1] = prefix[k] + tile_cost(p.tiles[k].r1 - p.tiles[k].r0);
+        prefix[k + 1] = prefix[k] + tile_cost(p.tiles[k].r1 - p.tiles[k].r0, overhead);
     }
     p.total_cost = prefix.back();
     p.begin.assign(static_cast<size_t>(xcds) + 1, 0);
@@ -374,34 +395,34 @@ inline QueuePlan plan_queues(std::vector<PlanInst> insts, int rows, int queues =
 // XCD's waves, plus half a tile for the end, where the last waves finish their last tiles at
 // staggered times.
 inline long long queue_model_cost(const std::vector<PlanInst>& insts, int rows, int waves,
-                                  int wpb = 4)
+                                  int wpb = 4, int overhead = 5)
 {
     waves = std::max(1, waves);
     const int queues = launch_queues(std::max(1, waves / std::max(1, wpb)));
-    const QueuePlan p = plan_queues(insts, rows, queues);
+    const QueuePlan p = plan_queues(insts, rows, queues, overhead);
     const long long per_xcd = std::max(1, waves / queues);
-    return (p.max_queue_cost + per_xcd - 1) / per_xcd + tile_cost(std::max(1, rows)) / 2;
+    return (p.max_queue_cost + per_xcd - 1) / per_xcd + tile_cost(std::max(1, rows), overhead) / 2;
 }
 
 // The two hand-outs as the load model and the timed choice see them.
 enum class HandOut { kStatic, kQueued };
 
 inline long long handout_cost(const std::vector<PlanInst>& insts, int rows, int waves,
-                              HandOut how)
+                              HandOut how, int overhead = 5)
 {
-    return how == HandOut::kQueued ? queue_model_cost(insts, rows, waves)
+    return how == HandOut::kQueued ? queue_model_cost(insts, rows, waves, 4, overhead)
                                    : fixed_max_cost(insts, rows, waves);
 }
 
 // model_rows and shortlist_rows for either hand-out (the static one: the busiest wave of the
 // round-robin plan; the queued one: queue_model_cost).
 inline int model_rows_for(const std::vector<PlanInst>& insts, int waves,
-                          const std::vector<int>& candidates, HandOut how)
+                          const std::vector<int>& candidates, HandOut how, int overhead = 5)
 {
     int best = candidates.empty() ? 32 : candidates.front();
     long long best_cost = std::numeric_limits<long long>::max();
     for (const int rows : candidates) {
-        const long long c = handout_cost(insts, rows, waves, how);
+        const long long c = handout_cost(insts, rows, waves, how, overhead);
         if (c < best_cost || (c == best_cost && rows > best)) {
             best_cost = c;
             best = rows;
@@ -409,6 +430,18 @@ inline int model_rows_for(const std::vector<PlanInst>& insts, int waves,
     }
     return best;
 }
+
+// The untimed choice (the default partition, re-made at every change of the stepping set):
+// the queued model with a tile's fixed cost at 14 half-rows and lengths from 12 rows. Measured
+// on FP32 sets of 1-8 M cells (one to thirty instances of 64^2 .. 1024^2), tiles of 8 rows
+// were the slowest or next to it every time, and at 5 half-rows the model picked 8-12 rows
+// where 16-24 ran 5-7 % faster (the served request stream: 17,930 Mcell-updates/s at its
+// picks against 19,000-19,100 at a fixed 16 or 24). At 14 it picks what ran fastest on each
+// measured set (12 rows for 1-2 M cells, 16 for three 1024^2 grids, 24 for 8 M cells of
+// random sizes). The timed choice's shortlist keeps 5, which ranks the mixed ensemble's
+// lengths as measured (16 first).
+constexpr int kPickTileOverhead = 14;
+constexpr int kPickMinRows = 12;
 
 inline std::vector<int> shortlist_rows_for(const std::vector<PlanLaunch>& launches,
                                            const std::vector<int>& candidates, HandOut how,

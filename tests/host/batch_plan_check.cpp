@@ -166,6 +166,39 @@ int main()
             CHECK(two == sl, "two equal launches rank alike");
         }
     }
+    // even segments: a strip of n rows in ceil(n / rows) tiles whose lengths differ by at most
+    // one, every row in exactly one of them
+    for (int n = 1; n <= 300; ++n) {
+        for (int rows = 1; rows <= 40; ++rows) {
+            const int k = segment_count(n, rows);
+            int lo = n, hi = 0, covered = 0;
+            for (int j = 0; j < k; ++j) {
+                const int len = segment_start(n, k, j + 1) - segment_start(n, k, j);
+                lo = std::min(lo, len);
+                hi = std::max(hi, len);
+                covered += len;
+            }
+            CHECK(segment_start(n, k, 0) == 0 && covered == n && hi <= rows && hi - lo <= 1 &&
+                      lo >= 1,
+                  "segments of %d rows at %d: lengths %d..%d", n, rows, lo, hi);
+        }
+    }
+    // the untimed choice on the queued hand-out (4096 waves): 12 rows for a set that fills
+    // the waves less than twice, longer for more cells, never below kPickMinRows
+    {
+        std::vector<int> cands;
+        for (const int r : {8, 12, 16, 20, 24, 28, 32, 40, 48, 64, 96, 128}) {
+            if (r >= kPickMinRows) cands.push_back(r);
+        }
+        auto pick = [&](const std::vector<PlanInst>& set) {
+            return model_rows_for(set, 4096, cands, HandOut::kQueued, kPickTileOverhead);
+        };
+        CHECK(pick({PlanInst{0, 1024, 1024}}) == 12, "one 1024^2: %d", pick({PlanInst{0, 1024, 1024}}));
+        std::vector<PlanInst> big;
+        for (int i = 0; i < 8; ++i) big.push_back(PlanInst{i, 1000, 1000});
+        const int r8 = pick(big);
+        CHECK(r8 >= 16 && r8 <= 32, "eight 1000^2: %d", r8);
+    }
     // replay accounting
     {
         const std::vector<int> before = {0, 16, 5, 9};
