@@ -136,6 +136,14 @@ void k_batch_stage(BatchTables tb, int parity)
             atomicMax(&tb.ctrl[held].chmax[parity ^ 1], held_bits);
         }
     };
+    // Queued: the waves of a workgroup take neighbouring tiles together (a band's strips come
+    // one after another in the queue), first the ones at their own indices, then one ticket of
+    // kWavesPerBlock tiles per round, drawn by wave 0 and passed on through LDS (two slots,
+    // by round parity, so a slot is rewritten only after every wave has read it). The four
+    // waves then march the same rows of neighbouring strips at about the same time, so the
+    // cache lines their strips share (halo lanes, the partial lines at strip edges) meet in
+    // L2 instead of being fetched and written twice.
+    __shared__ int block_ticket[2];
     // stage 2 takes the wave's tiles (the queue's) in reverse order (and marches each upward),
     // so that it starts on what stage 1 touched last, and the next stage 1 on what stage 2
     // touched last
@@ -144,14 +152,18 @@ void k_batch_stage(BatchTables tb, int parity)
         if (queued) {
             int u = lw;
             if (j > 0) {
-                int v = 0;
-                if (lane == 0) {
-                    v = atomicAdd(&tb.next[q], 1);
+                if (wave == 0 && lane == 0) {
+                    block_ticket[j & 1] = atomicAdd(&tb.next[q], kWavesPerBlock);
                 }
-                u = wq + uniform_i32(v);
+                __syncthreads();
+                const int base = wq + uniform_i32(block_ticket[j & 1]);
+                if (base >= qn) {
+                    break;  // the same for every wave of the workgroup
+                }
+                u = base + wave;
             }
             if (u >= qn) {
-                break;
+                continue;  // the queue ends inside this round: the next one ends the loop
             }
             t = qb + (STAGE == 2 ? qn - 1 - u : u);
         } else {

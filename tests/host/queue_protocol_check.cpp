@@ -1,12 +1,13 @@
 // CPU model of the batched engine's dynamic tile hand-out (mhd_gpu_solver_amd/csrc/
 // mhd_batch_plan.hpp, plan_queues; the ticket loop of k_batch_stage in mhd_batch_kernels.hpp).
 //
-// Waves of a launch run in a random interleaving. A wave first takes the tile at its own index
-// among its XCD's waves, then tickets from its XCD's counter (an atomic add, offset by the
-// XCD's wave count) until a ticket is past the queue's end. At its start, each stage's kernel
-// zeroes the other stage's counters. Checked over long random sequences of launches whose
-// tile lists and grids change between launches: every launch marches every tile exactly once,
-// and every wave's loop ends.
+// Workgroups of a launch run in a random interleaving. The waves of a workgroup first take
+// the tiles at their own indices among their XCD's waves, then, round by round, one ticket of
+// four tiles from their XCD's counter (an atomic add of 4, offset by the XCD's wave count;
+// wave w marches the ticket's tile w), until a ticket starts past the queue's end. At its
+// start, each stage's kernel zeroes the other stage's counters. Checked over long random
+// sequences of launches whose tile lists and grids change between launches: every launch
+// marches every tile exactly once, and every workgroup's loop ends.
 //
 // Also modelled: the protocol that hung in an earlier build (counters zeroed by the last wave
 // of each launch, counted on the way out, all tiles from tickets), with a wave that returns
@@ -39,23 +40,19 @@ struct Device {
 };
 
 // One launch of stage `stage` (1 or 2) over `plan` with `blocks` workgroups of 4 waves.
-// Returns how many times each tile was marched; `steps` counts the waves' loop iterations.
+// Returns how many times each tile was marched; `steps` counts the tiles marched.
 static std::vector<int> launch(Device& d, const QueuePlan& plan, int stage, int blocks,
                                std::mt19937& rng, long long* steps)
 {
     const int wpb = 4;
-    struct Wave {
-        int x, lw, taken;
-        bool done;
-    };
-    std::vector<Wave> waves;
     const int nq = launch_queues(blocks);
     CHECK(static_cast<int>(plan.begin.size()) == nq + 1, "plan for %d queues", nq);
-    for (int b = 0; b < blocks; ++b) {
-        for (int w = 0; w < wpb; ++w) {
-            waves.push_back(Wave{b % nq, queue_wave_index(blocks, b, w, wpb), 0, false});
-        }
-    }
+    struct Block {
+        int x, round;
+        bool done;
+    };
+    std::vector<Block> bl;
+    for (int b = 0; b < blocks; ++b) bl.push_back(Block{b % nq, 0, false});
     std::vector<int> hits(plan.tiles.size(), 0);
     const int other = stage == 1 ? 1 : 0;  // index of the other stage's counters
     const int mine = stage - 1;
@@ -63,28 +60,42 @@ static std::vector<int> launch(Device& d, const QueuePlan& plan, int stage, int 
     const long long reset_at = std::uniform_int_distribution<long long>(0, 50)(rng);
     bool reset_done = false;
     long long iter = 0;
-    size_t live = waves.size();
+    size_t live = bl.size();
+    auto march = [&](const Block& k, int u) {
+        const int qb = plan.begin[static_cast<size_t>(k.x)];
+        const int qn = plan.begin[static_cast<size_t>(k.x) + 1] - qb;
+        if (u >= qn) return;  // this wave has no tile in the round
+        const int t = qb + (stage == 2 ? qn - 1 - u : u);
+        ++hits[static_cast<size_t>(t)];
+        ++*steps;
+    };
     while (live > 0) {
         if (!reset_done && iter >= reset_at) {
             for (int x = 0; x < kPlanXcds; ++x) d.next[other][x] = 0;
             reset_done = true;
         }
-        Wave& w = waves[std::uniform_int_distribution<size_t>(0, waves.size() - 1)(rng)];
+        const size_t bi = std::uniform_int_distribution<size_t>(0, bl.size() - 1)(rng);
+        Block& k = bl[bi];
         ++iter;
-        if (w.done) continue;
-        const int qb = plan.begin[static_cast<size_t>(w.x)];
-        const int qn = plan.begin[static_cast<size_t>(w.x) + 1] - qb;
-        const int wx = queue_waves(blocks, w.x, wpb);
-        const int u = w.taken == 0 ? w.lw : wx + d.next[mine][w.x]++;
-        ++w.taken;
-        if (u >= qn) {
-            w.done = true;
-            --live;
-            continue;
+        if (k.done) continue;
+        const int qn = plan.begin[static_cast<size_t>(k.x) + 1] - plan.begin[static_cast<size_t>(k.x)];
+        if (k.round == 0) {
+            // the waves' own tiles
+            for (int w = 0; w < wpb; ++w) {
+                march(k, queue_wave_index(blocks, static_cast<int>(bi), w, wpb));
+            }
+        } else {
+            // one ticket of wpb tiles for the workgroup, drawn after its waves met
+            const int base = queue_waves(blocks, k.x, wpb) + d.next[mine][k.x];
+            d.next[mine][k.x] += wpb;
+            if (base >= qn) {
+                k.done = true;
+                --live;
+                continue;
+            }
+            for (int w = 0; w < wpb; ++w) march(k, base + w);
         }
-        const int t = qb + (stage == 2 ? qn - 1 - u : u);
-        ++hits[static_cast<size_t>(t)];
-        ++*steps;
+        ++k.round;
     }
     if (!reset_done) {
         for (int x = 0; x < kPlanXcds; ++x) d.next[other][x] = 0;
