@@ -1040,17 +1040,19 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageBuffers b, Geom g, Physic
     if (!st.active) {
         return;
     }
-    // block -> (strip, segment group); its four waves march four consecutive row segments.
-    // Consecutive logical blocks (neighbouring strips) are placed on the same XCD so that the
-    // halo columns and the cache lines shared across a strip edge meet in one L2.
+    // Tiles in row-band order (a band's strips left to right), four consecutive ones per block:
+    // the four waves march the same rows of neighbouring strips at about the same time, so the
+    // halo columns and the cache lines shared across their strip edges meet in L2. Consecutive
+    // logical blocks are placed on the same XCD, which thus walks down a run of bands.
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = wave_in_block();
     // (stage 2 takes each XCD's blocks in reverse order, so that it starts on the tiles that
     // stage 1 touched last)
     const int logical = xcd_logical_block(blockIdx.x, gridDim.x, STAGE == 2);
-    const int strip = logical % tl.strips;
-    const int seg = (logical / tl.strips) * kWavesPerBlock + wave;
-    if (strip >= tl.strips || seg >= tl.segs) {
+    const long long wl = static_cast<long long>(logical) * kWavesPerBlock + wave;
+    const int strip = static_cast<int>(wl % tl.strips);
+    const int seg = static_cast<int>(wl / tl.strips);
+    if (seg >= tl.segs) {
         return;
     }
     const int r0 = g.ng + seg * tl.seg_rows;

@@ -11,9 +11,10 @@ namespace detail {
 
 inline dim3 stage_grid(const Tiling& tl)
 {
-    // one block per (strip, group of four row segments); flat grid, remapped in the kernel
-    const long long groups = (tl.segs + kWavesPerBlock - 1) / kWavesPerBlock;
-    return dim3(static_cast<unsigned>(tl.strips * groups), 1, 1);
+    // one wave per (strip, row segment), four per block in row-band order; flat grid,
+    // remapped in the kernel
+    const long long tiles = static_cast<long long>(tl.strips) * tl.segs;
+    return dim3(static_cast<unsigned>((tiles + kWavesPerBlock - 1) / kWavesPerBlock), 1, 1);
 }
 
 inline unsigned flat_blocks(long long work, int block, int cap)
