@@ -1049,9 +1049,10 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageBuffers b, Geom g, Physic
     // (stage 2 takes each XCD's blocks in reverse order, so that it starts on the tiles that
     // stage 1 touched last)
     const int logical = xcd_logical_block(blockIdx.x, gridDim.x, STAGE == 2);
-    const long long wl = static_cast<long long>(logical) * kWavesPerBlock + wave;
-    const int strip = static_cast<int>(wl % tl.strips);
-    const int seg = static_cast<int>(wl / tl.strips);
+    // (the host caps a grid at 2^31 tiles, so 32-bit unsigned arithmetic holds)
+    const unsigned wl = static_cast<unsigned>(logical) * kWavesPerBlock + wave;
+    const int strip = static_cast<int>(wl % static_cast<unsigned>(tl.strips));
+    const int seg = static_cast<int>(wl / static_cast<unsigned>(tl.strips));
     if (seg >= tl.segs) {
         return;
     }

@@ -27,6 +27,10 @@ template <int STAGE, int LIM, int NI, int REMAP, typename R, int FAST>
 hipError_t launch_one(const StageBuffers& b, const Geom& g, const Physics& ph, const Tiling& tl,
                       int parity, hipStream_t s)
 {
+    // the kernel numbers its tiles in 32 bits (a grid of 2^31 tiles would hold over 10^11 cells)
+    if (static_cast<long long>(tl.strips) * tl.segs > (1ll << 31)) {
+        return hipErrorInvalidValue;
+    }
     hipLaunchKernelGGL((k_stage<STAGE, LIM, NI, REMAP, R, FAST>), stage_grid(tl), dim3(kBlock), 0, s, b,
                        g, ph, tl, parity);
     return hipGetLastError();
