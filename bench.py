@@ -44,9 +44,9 @@ def log(msg):
 
 # ---- single large run (Orszag-Tang N^2) ----------------------------------------------------
 
-def make_device(m, nx, ny, precision, bc=(0, 0)):
+def make_device(m, nx, ny, precision, bc=(0, 0), y_max=1.0):
     from mhd_gpu_solver_amd import _lib
-    grid = _lib.Grid(nx, ny, 2, 0.0, 1.0, 0.0, 1.0)
+    grid = _lib.Grid(nx, ny, 2, 0.0, 1.0, 0.0, y_max)
     dev = m.DeviceSolver(grid, _lib.PRECISIONS[precision])
     dev.configure(bc[0], bc[1], None, True)
     return dev
@@ -61,9 +61,11 @@ def params(m, gamma=5.0 / 3.0):
     return p
 
 
-def single_leg(m, n, precision, steps, warmup, U0):
-    """`steps` timed steps of the single-run solver after `warmup`."""
-    dev = make_device(m, n, n, precision)
+def single_leg(m, n, precision, steps, warmup, U0, ny=None, bc=(0, 0), y_max=1.0):
+    """`steps` timed steps of the single-run solver after `warmup` (an n x n grid, or n x ny
+    over [0, 1] x [0, y_max])."""
+    ny = n if ny is None else ny
+    dev = make_device(m, n, ny, precision, bc, y_max)
     t0 = time.perf_counter()
     dev.upload(U0)                                   # full fp64 host state, ghosts included
     upload_s = time.perf_counter() - t0
@@ -91,7 +93,7 @@ def single_leg(m, n, precision, steps, warmup, U0):
     download_s = time.perf_counter() - t0
     if not np.isfinite(U).all():
         raise RuntimeError(f"{precision}: non-finite state after the timed steps")
-    cells = n * n
+    cells = n * ny
     # what a caller handing over host buffers sees for one call of `steps` steps: upload,
     # graph-path steps, download (MHDSolver.run moves the state once each way per call)
     pcie = {"mcups": round(cells * steps / (upload_s + graph_s + download_s) / 1e6, 2),
@@ -580,6 +582,25 @@ def main():
                                      ceiling=ceilings[eb]),
             }
         out["single_run"] = {"workload": f"orszag_tang_{n}x{n}_single_run", **legs}
+        # One GPU's share of the same run split into 8 y-slabs: a n x n/8 slab of the state
+        # (its ghost rows the neighbouring slabs' rows at t = 0), periodic in x; outflow in y
+        # stands in for the halo rows a multi-GPU run would receive before each stage.
+        slabs = 8
+        if n % slabs == 0:
+            h = n // slabs
+            Us = np.ascontiguousarray(U0[:, :h + 4, :])
+            slab = {"shape": f"{n}x{h}", "of": slabs, "bc": "periodic x, outflow y"}
+            for prec, eb in (("fp64", 8), ("fp32", 4)):
+                leg = single_leg(m, n, prec, args.single_steps, args.warmup, Us, ny=h,
+                                 bc=(0, 2), y_max=h / n)
+                log(f"slab {n}x{h} {prec}: {leg['mcups']:.1f} Mcell-updates/s")
+                slab[prec] = {
+                    "value": round(leg["mcups"], 2), "ms_per_step": round(leg["ms_per_step"], 4),
+                    "seg_rows": leg["seg_rows"],
+                    "roofline": roofline(leg["kernels"], leg["cells"], leg["ms_per_step"], eb,
+                                         "k_stage"),
+                }
+            out["single_run"]["slab"] = slab
         if args.only == "single":
             out.update({"value": legs["fp64"]["value"], "ms_per_step": legs["fp64"]["ms_per_step"],
                         "dtype": "f64", "config": {"workload": f"orszag_tang_{n}x{n}_single_run"},
