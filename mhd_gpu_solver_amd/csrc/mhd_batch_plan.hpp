@@ -292,6 +292,25 @@ inline WavePlan plan_fill(const std::vector<PlanInst>& insts, int waves, int min
 // (the stages alternate on one stream), so no wave has to know that it is the last one.
 constexpr int kPlanXcds = 8;
 
+// Workgroups are dispatched round-robin over the XCDs (physical id p runs on XCD p % 8). The
+// logical id of workgroup p of nb: each XCD receives a contiguous range of logical ids, in the
+// order its workgroups are dispatched (reversed within each XCD with `reverse`). (constexpr:
+// the stage kernels use it.)
+constexpr int xcd_logical_block(int p, int nb, bool reverse = false)
+{
+    const int q = nb / kPlanXcds;
+    const int r = nb % kPlanXcds;
+    const int xcd = p % kPlanXcds;
+    const int n = xcd < r ? q + 1 : q;  // workgroups on this XCD
+    const int local = reverse ? n - 1 - p / kPlanXcds : p / kPlanXcds;
+    return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
+}
+static_assert(xcd_logical_block(0, 20) == 0 && xcd_logical_block(8, 20) == 1 &&
+                  xcd_logical_block(1, 20) == 3 && xcd_logical_block(19, 20) == 11 &&
+                  xcd_logical_block(15, 20) == 19 &&
+                  xcd_logical_block(0, 20, true) == 2,
+              "XCD-contiguous logical workgroups");
+
 // Queues of a launch of `blocks` workgroups: one per XCD, or one per workgroup when there are
 // fewer workgroups than XCDs (every queue needs a wave). Workgroup b takes queue b mod
 // queues, which is its XCD (workgroups are dispatched round-robin over the XCDs).

@@ -16,6 +16,7 @@
 // are no-ops and a batch can run ahead of the host.
 #pragma once
 
+#include "mhd_batch_plan.hpp"
 #include "mhd_physics.hpp"
 
 #include <hip/hip_runtime.h>
@@ -177,21 +178,6 @@ __device__ __forceinline__ StepState step_state(const Ctrl* ctrl, const Physics&
 __device__ __forceinline__ bool ctrl_pending(const Ctrl* ctrl, const Physics& ph, int p)
 {
     return ctrl->t[p] < ph.t_end && ctrl->steps[p] < ph.max_steps;
-}
-
-// Workgroups are dispatched round-robin over the 8 XCDs (physical id p runs on XCD p % 8).
-// Returns a logical id such that each XCD receives a contiguous range of logical ids, in the
-// order its workgroups are dispatched.
-constexpr int kXcds = 8;
-
-__device__ __forceinline__ int xcd_logical_block(int p, int nb, bool reverse = false)
-{
-    const int q = nb / kXcds;
-    const int r = nb % kXcds;
-    const int xcd = p % kXcds;
-    const int n = xcd < r ? q + 1 : q;  // blocks on this XCD
-    const int local = reverse ? n - 1 - p / kXcds : p / kXcds;
-    return xcd < r ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
 }
 
 template <typename R>

@@ -199,6 +199,52 @@ int main()
         const int r8 = pick(big);
         CHECK(r8 >= 16 && r8 <= 32, "eight 1000^2: %d", r8);
     }
+    // the XCD remap is a permutation of the workgroups, each XCD's logical ids contiguous
+    for (int nb = 1; nb <= 3000; nb += (nb < 100 ? 1 : 37)) {
+        for (const bool rev : {false, true}) {
+            std::vector<int> seen(static_cast<size_t>(nb), 0);
+            std::vector<int> lo(kPlanXcds, nb), hi(kPlanXcds, -1), cnt(kPlanXcds, 0);
+            for (int p = 0; p < nb; ++p) {
+                const int l = xcd_logical_block(p, nb, rev);
+                CHECK(l >= 0 && l < nb, "remap of %d of %d: %d", p, nb, l);
+                if (l < 0 || l >= nb) continue;
+                ++seen[static_cast<size_t>(l)];
+                const int x = p % kPlanXcds;
+                lo[x] = std::min(lo[x], l);
+                hi[x] = std::max(hi[x], l);
+                ++cnt[x];
+            }
+            for (int v : seen) CHECK(v == 1, "remap of %d workgroups not a permutation", nb);
+            for (int x = 0; x < kPlanXcds; ++x) {
+                CHECK(cnt[x] == 0 || hi[x] - lo[x] + 1 == cnt[x], "XCD %d of %d not contiguous",
+                      x, nb);
+            }
+        }
+    }
+    // the single-run grid (k_stage): one wave per (strip, row segment) in row-band order,
+    // four per workgroup; every tile marched by exactly one wave
+    for (int strips = 1; strips <= 70; strips += 3) {
+        for (int segs = 1; segs <= 90; segs += 7) {
+            const long long tiles = static_cast<long long>(strips) * segs;
+            const int nb = static_cast<int>((tiles + 3) / 4);
+            std::vector<int> hit(static_cast<size_t>(tiles), 0);
+            for (const bool rev : {false, true}) {
+                std::fill(hit.begin(), hit.end(), 0);
+                for (int p = 0; p < nb; ++p) {
+                    for (int w = 0; w < 4; ++w) {
+                        const unsigned wl =
+                            static_cast<unsigned>(xcd_logical_block(p, nb, rev)) * 4 + w;
+                        const int strip = static_cast<int>(wl % static_cast<unsigned>(strips));
+                        const int seg = static_cast<int>(wl / static_cast<unsigned>(strips));
+                        if (seg >= segs) continue;
+                        ++hit[static_cast<size_t>(seg) * strips + strip];
+                    }
+                }
+                for (int h : hit) CHECK(h == 1, "k_stage grid %d x %d: a tile marched %d times",
+                                        strips, segs, h);
+            }
+        }
+    }
     // replay accounting
     {
         const std::vector<int> before = {0, 16, 5, 9};
