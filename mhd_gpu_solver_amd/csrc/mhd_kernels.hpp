@@ -1051,8 +1051,14 @@ __global__ __launch_bounds__(kBlock) void k_stage(StageBuffers b, Geom g, Physic
     // wave-private ring of the last three raw (conserved) rows: the RK update of row c-1 reads
     // its stage input from here instead of global memory
     __shared__ R ring[kWavesPerBlock][3][kNC][kWave];
-    const R local_speed = march_tile<STAGE, LIM, NI, REMAP, 0, 1, kMarchDir<STAGE>, R>(
-        src, u0, dst, b.mask, g, ph, st, strip, r0, r1, 0, r1, ring[wave], lane);
+    // Odd row segments march the other way: the two rows each side of a segment boundary are
+    // read by both segments' waves, and with the directions alternating both read them at the
+    // same end of their march (at about the same time), so the second read finds them in L2.
+    const R local_speed =
+        (seg & 1) ? march_tile<STAGE, LIM, NI, REMAP, 0, 1, -kMarchDir<STAGE>, R>(
+                        src, u0, dst, b.mask, g, ph, st, strip, r0, r1, 0, r1, ring[wave], lane)
+                  : march_tile<STAGE, LIM, NI, REMAP, 0, 1, kMarchDir<STAGE>, R>(
+                        src, u0, dst, b.mask, g, ph, st, strip, r0, r1, 0, r1, ring[wave], lane);
     if (STAGE == 2) {
         const R m = wave_max(local_speed);
         if (lane == 0) {
