@@ -5,8 +5,14 @@ file, so that it travels with the source tree and is the object the package load
 
 * mhd_fp64.hip, mhd_solver.hip, mhd_host.cpp, ... : -ffp-contract=off (fp64 parity with the
   CPU path depends on every product and sum rounding separately)
-* mhd_fp32.hip (single-run and batched FP32 kernels): contraction allowed (FP32-bulk path)
-* mhd_fp64fast.hip (the fast fp64 stepping kernels): contraction allowed
+* mhd_fp32.hip (single-run FP32 kernels) and mhd_fp64fast.hip (the fast fp64 stepping
+  kernels): contraction within each source expression (-ffp-contract=on), plus the fused
+  multiply-adds the sources spell out. Not -ffp-contract=fast: across expressions the compiler
+  picks which product of a sum to fuse from the code around it, which differs between the two
+  directions these kernels march row segments in, and the results must not depend on the
+  direction (nor, through it, on the tiling).
+* mhd_fp32batch.hip (the batch engine's FP32 kernels, which march in one direction):
+  contraction across expressions (-ffp-contract=fast).
 * all HIP code: packed fp32 instructions disabled (-packed-fp32-ops)
 """
 from __future__ import annotations
@@ -31,7 +37,8 @@ COMMON = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
-CONTRACT_FAST = {"mhd_fp32.hip", "mhd_fp64fast.hip"}
+CONTRACT_ON = {"mhd_fp32.hip", "mhd_fp64fast.hip"}
+CONTRACT_FAST = {"mhd_fp32batch.hip"}
 
 
 def hipcc() -> str:
@@ -47,7 +54,8 @@ def sources():
 
 def _compile(src: pathlib.Path) -> pathlib.Path:
     obj = OBJDIR / (src.name + ".o")
-    contract = "-ffp-contract=fast" if src.name in CONTRACT_FAST else "-ffp-contract=off"
+    contract = ("-ffp-contract=fast" if src.name in CONTRACT_FAST else
+                "-ffp-contract=on" if src.name in CONTRACT_ON else "-ffp-contract=off")
     lang = ["-x", "hip"] if src.suffix == ".hip" else []
     cmd = [hipcc(), *COMMON, contract, *lang, "-c", str(src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -212,9 +212,17 @@ void k_batch_stage(BatchTables tb, int parity)
         const int dB = uniform_i32(tile.r0b < 0 ? 0 : tile.r0b - tile.r0);
         const int r1b = uniform_i32(tile.r0b < 0 ? tile.r1 : tile.r1b);
         const int strip = uniform_i32(tile.strip);
-        const R local = march_tile<STAGE, LIM, NI, REMAP, 2, 1, kMarchDir<STAGE>, R>(
-            src, u0, dst, REMAP ? nullptr : in.mask, g, in.ph, st, strip, r0, r1, dB, r1b,
-            ring[wave], lane);
+        // odd bands march the other way (see plan_queues) in the fp64 kernels; the FP32 kernel
+        // marches every tile one way: with both directions inlined its stage 2 spills
+        // registers and ran 1.5 % slower on the ensemble
+        const R local =
+            (sizeof(R) == 8 && uniform_i32(tile.flip))
+                ? march_tile<STAGE, LIM, NI, REMAP, 2, 1, -kMarchDir<STAGE>, R>(
+                      src, u0, dst, REMAP ? nullptr : in.mask, g, in.ph, st, strip, r0, r1, dB,
+                      r1b, ring[wave], lane)
+                : march_tile<STAGE, LIM, NI, REMAP, 2, 1, kMarchDir<STAGE>, R>(
+                      src, u0, dst, REMAP ? nullptr : in.mask, g, in.ph, st, strip, r0, r1, dB,
+                      r1b, ring[wave], lane);
         if (STAGE == 2 && !tb.dry) {
             const int inst = uniform_i32(tile.inst);
             if (inst != held) {

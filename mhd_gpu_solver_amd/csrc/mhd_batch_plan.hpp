@@ -25,6 +25,7 @@ struct BatchTile {
     int r0, r1;   // global rows [r0, r1) marched by the wave (by its lower half when r0b >= 0)
     int r0b, r1b; // half-wave tile of a narrow strip: the upper half's rows start at r0b and
                   // are updated below r1b; r0b < 0 for a whole-wave tile
+    int flip = 0; // march the rows in the other direction (the odd bands of an instance)
 };
 
 // One instance as the planner sees it: its slot and interior size.
@@ -356,9 +357,10 @@ static_assert(segment_count(128, 24) == 6 && segment_start(128, 6, 1) == 21 &&
 
 // Segments of at most `rows` rows, even within each strip, queued largest instance first (the
 // short tiles of the small grids come last, where the hand-out ends), each instance band by
-// band, a band's strips left to right (its half-wave tile last); the list is cut into
-// `queues` contiguous queues of about equal load-model cost (launch_queues of the launch's
-// grid).
+// band, a band's strips left to right (its half-wave tile last); the tiles of odd bands march
+// the other way (`flip`), so that the rows on each side of a band boundary are loaded by both
+// bands' waves at the same end of their march. The list is cut into `queues` contiguous
+// queues of about equal load-model cost (launch_queues of the launch's grid).
 inline QueuePlan plan_queues(std::vector<PlanInst> insts, int rows, int queues = kPlanXcds,
                              int overhead = 5)
 {
@@ -377,11 +379,13 @@ inline QueuePlan plan_queues(std::vector<PlanInst> insts, int rows, int queues =
             const int a1 = segment_start(in.ny, kw, j + 1);
             for (int st = 0; st < s.wide; ++st) {
                 p.tiles.push_back(make_tile(in.slot, in.nx, in.ny, st, a0, a1));
+                p.tiles.back().flip = j & 1;
             }
             if (j < kh) {
                 p.tiles.push_back(make_tile(in.slot, in.nx, in.ny, s.wide,
                                             segment_start(s.half_rows, kh, j),
                                             segment_start(s.half_rows, kh, j + 1)));
+                p.tiles.back().flip = j & 1;
             }
         }
     }

@@ -702,6 +702,46 @@ __device__ __forceinline__ R face_state(R w, R d)
     }
 }
 
+// A cell's right-hand side from its flux differences, -dF (1/dx) - dG (1/dy)
+// (include/mhd/rhs.hpp:116-123), and the two RK combines u + dt L and 1/2 (u0 + u) + dt/2 L
+// (include/mhd/field_ops.hpp:47-63). The contracted builds take their fused multiply-adds
+// from the source: left to the compiler, which of the two products it fused depended on the
+// direction the rows were marched in, and the two directions must give the same bits.
+template <typename R>
+__device__ __forceinline__ R cell_rhs(R dF, R dG, R inv_dx, R inv_dy)
+{
+    if constexpr (sizeof(R) == 4) {
+        return __builtin_fmaf(-dG, inv_dy, -(dF * inv_dx));
+    } else if constexpr (MHD_FAST_FP64) {
+        return __builtin_fma(-dG, inv_dy, -(dF * inv_dx));
+    } else {
+        return -dF * inv_dx - dG * inv_dy;
+    }
+}
+template <typename R>
+__device__ __forceinline__ R rk_stage1(R u, R dt, R l)
+{
+    if constexpr (sizeof(R) == 4) {
+        return __builtin_fmaf(dt, l, u);
+    } else if constexpr (MHD_FAST_FP64) {
+        return __builtin_fma(dt, l, u);
+    } else {
+        return u + dt * l;
+    }
+}
+template <typename R>
+__device__ __forceinline__ R rk_stage2(R u0, R u, R half_dt, R l)
+{
+    const R half(0.5);
+    if constexpr (sizeof(R) == 4) {
+        return __builtin_fmaf(half_dt, l, half * (u0 + u));
+    } else if constexpr (MHD_FAST_FP64) {
+        return __builtin_fma(half_dt, l, half * (u0 + u));
+    } else {
+        return half * (u0 + u) + half_dt * l;
+    }
+}
+
 // Stage 1 marches its rows in increasing order and stage 2 in decreasing order, so that each
 // stage starts on the rows the previous one touched last (still in the memory-side cache).
 template <int STAGE>
@@ -937,7 +977,7 @@ __device__ __forceinline__ R march_tile(const Planes& src, const Planes& u0, con
                 for (int k = 0; k < kNC; ++k) {
                     const R Fr = from_right(F[k]);
                     // include/mhd/rhs.hpp:116-123
-                    L[k] = -(Fr - F[k]) * inv_dx - (Gup[k] - Glo[k]) * inv_dy;
+                    L[k] = cell_rhs(Fr - F[k], Gup[k] - Glo[k], inv_dx, inv_dy);
                 }
                 if (psi_damp != R(0)) {
                     L[PSI] -= psi_damp * Wa[PSI];
@@ -957,7 +997,8 @@ __device__ __forceinline__ R march_tile(const Planes& src, const Planes& u0, con
                         // include/mhd/field_ops.hpp:47-53
 #pragma unroll
                         for (int k = 0; k < kNC; ++k) {
-                            dst.st(k, rr, g.nxt, cell_boff, ring[slot][k][lane] + dt * L[k]);
+                            dst.st(k, rr, g.nxt, cell_boff,
+                                   rk_stage1(ring[slot][k][lane], dt, L[k]));
                         }
                     } else {
                         // include/mhd/field_ops.hpp:55-63
@@ -965,7 +1006,7 @@ __device__ __forceinline__ R march_tile(const Planes& src, const Planes& u0, con
 #pragma unroll
                         for (int k = 0; k < kNC; ++k) {
                             const R u0k = early_u0 ? U0r[k] : u0.ld(k, rr, g.nxt, cell_boff);
-                            un[k] = half * (u0k + ring[slot][k][lane]) + half_dt * L[k];
+                            un[k] = rk_stage2(u0k, ring[slot][k][lane], half_dt, L[k]);
                         }
 #pragma unroll
                         for (int k = 0; k < kNC; ++k) {

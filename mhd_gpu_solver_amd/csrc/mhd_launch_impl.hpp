@@ -247,7 +247,10 @@ int stage_blocks_per_cu(int limiter, bool nonideal, bool remap)
     return (e == hipSuccess && n > 0) ? n : 1;
 }
 
-#define MHDHIP_INSTANTIATE(R)                                                                    \
+// The single-run launchers of one precision, and the batch engine's (separate, so that a
+// precision's two sets can be compiled with different contraction flags).
+#define MHDHIP_INSTANTIATE(R) MHDHIP_INSTANTIATE_SINGLE(R) MHDHIP_INSTANTIATE_BATCH(R)
+#define MHDHIP_INSTANTIATE_SINGLE(R)                                                             \
     template hipError_t launch_ctrl_init<R>(Ctrl*, hipStream_t);                                 \
     template hipError_t launch_cfl<R>(const R*, const Geom&, const Physics&, const uint8_t*,     \
                                       Ctrl*, hipStream_t);                                       \
@@ -256,14 +259,15 @@ int stage_blocks_per_cu(int limiter, bool nonideal, bool remap)
                                             const Gate&, hipStream_t);                           \
     template hipError_t launch_stage<R>(int, int, bool, bool, const StageBuffers&, const Geom&,  \
                                         const Physics&, const Tiling&, int, hipStream_t);         \
-    template hipError_t launch_batch_stage<R>(int, int, bool, bool, const BatchTables&, int, int, \
-                                              hipStream_t);                                       \
-    template hipError_t launch_batch_fill<R>(const BatchTables&, int, int, int, hipStream_t);      \
-    template int batch_blocks_per_cu<R>(int, int, bool, bool);                                   \
     template int stage_blocks_per_cu<R>(int, bool, bool);                                        \
     template hipError_t launch_totals<R>(const R*, const Geom&, DD*, double*, hipStream_t);     \
     template hipError_t launch_max_div_b<R>(const R*, const Geom&, double, double,              \
                                             unsigned long long*, hipStream_t);
+#define MHDHIP_INSTANTIATE_BATCH(R)                                                              \
+    template hipError_t launch_batch_stage<R>(int, int, bool, bool, const BatchTables&, int, int, \
+                                              hipStream_t);                                       \
+    template hipError_t launch_batch_fill<R>(const BatchTables&, int, int, int, hipStream_t);      \
+    template int batch_blocks_per_cu<R>(int, int, bool, bool);
 
 // The stepping launchers of the fast fp64 build (mhd_fp64fast.hip).
 #define MHDHIP_INSTANTIATE_FAST(R)                                                               \

@@ -109,6 +109,18 @@ __device__ __forceinline__ double drcp(double x) { return 1.0 / x; }
 #endif
 __device__ __forceinline__ float dsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ float drcp(float x) { return __builtin_amdgcn_rcpf(x); }
+// a b + c: one fused multiply-add in the FP32 and contracted fp64 builds, the product and the
+// sum rounded separately in the exact fp64 build. The contracted builds fuse only within one
+// expression (-ffp-contract=on); a product and a sum in separate statements are joined here.
+__device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmadd(double a, double b, double c)
+{
+    if constexpr (MHD_FAST_FP64) {
+        return __builtin_fma(a, b, c);
+    } else {
+        return a * b + c;
+    }
+}
 __device__ __forceinline__ double dabs(double x) { return __builtin_fabs(x); }
 __device__ __forceinline__ float dabs(float x) { return __builtin_fabsf(x); }
 
@@ -190,10 +202,12 @@ struct MixedOps<float> {
         const double r = rho;
         double inv = inv_rho;
         inv = __builtin_fma(__builtin_fma(-r, inv, 1.0), inv, inv);
-        const double ekin = 0.5 * (dmx * dmx + dmy * dmy + dmz * dmz) * inv;
-        const double emag = 0.5 * (dbx * dbx + dby * dby + dbz * dbz);
-        return static_cast<float>(static_cast<double>(gm1) *
-                                  (static_cast<double>(e) - ekin - emag));
+        const double m2 = dmx * dmx + dmy * dmy + dmz * dmz;
+        const double b2 = dbx * dbx + dby * dby + dbz * dbz;
+        // e - m^2 / (2 rho) - B^2 / 2, each subtraction fused with its product
+        const double eint =
+            __builtin_fma(-0.5, b2, __builtin_fma(-0.5 * m2, inv, static_cast<double>(e)));
+        return static_cast<float>(static_cast<double>(gm1) * eint);
     }
 };
 
@@ -207,9 +221,8 @@ __device__ __forceinline__ R fast_speed(const R (&W)[kNC], R gamma)
     const R cs2 = gamma * W[PR] * inv_rho;
     const R bn2 = W[D::BN] * W[D::BN];
     const R b2 = bn2 + W[D::BT] * W[D::BT] + W[BZ] * W[BZ];
-    const R va2 = b2 * inv_rho;
     const R van2 = bn2 * inv_rho;
-    const R sum = cs2 + va2;
+    const R sum = fmadd(b2, inv_rho, cs2);  // cs2 + va2, va2 = B^2 / rho
     const R disc = MixedOps<R>::disc(sum, cs2, van2);
     const R cf2 = half * (sum + dsqrt(disc));
     return dsqrt(hmax(cf2, R(0)));
@@ -229,9 +242,9 @@ __device__ __forceinline__ void cons_flux(const R (&W)[kNC], R inv_gm1, R ch2, R
     const R mz = rho * vz;
     const R v2 = vn * vn + vt * vt + vz * vz;
     const R b2 = bn * bn + bt * bt + bz * bz;
-    const R pmag = half * b2;
-    const R energy = p * inv_gm1 + half * rho * v2 + pmag;
-    const R pstar = p + pmag;
+    // pmag = B^2 / 2 enters both sums below
+    const R energy = fmadd(half, b2, p * inv_gm1 + half * rho * v2);
+    const R pstar = fmadd(half, b2, p);
     const R vb = vn * bn + vt * bt + vz * bz;
 
     U[RHO] = rho;
