@@ -436,6 +436,9 @@ def main():
     ap.add_argument("--single-steps", type=int, default=100)
     ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-slab", action="store_true",
+                    help="skip the one-GPU slab of the single run (a profile's last launches "
+                         "of each single-run kernel are then the full grid's)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["ensemble", "single", "compare", "serve", "configs"],
                     default=None)
@@ -586,7 +589,7 @@ def main():
         # (its ghost rows the neighbouring slabs' rows at t = 0), periodic in x; outflow in y
         # stands in for the halo rows a multi-GPU run would receive before each stage.
         slabs = 8
-        if n % slabs == 0:
+        if n % slabs == 0 and not args.no_slab:
             h = n // slabs
             Us = np.ascontiguousarray(U0[:, :h + 4, :])
             slab = {"shape": f"{n}x{h}", "of": slabs, "bc": "periodic x, outflow y"}
