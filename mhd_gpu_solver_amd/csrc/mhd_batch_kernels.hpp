@@ -1,12 +1,13 @@
 // Batched stage kernel: many independent grids of different sizes advance in one persistent
 // launch per RK stage, each with its own dt, gamma, CFL number, boundary types and end time.
 //
-// The work list is a device table of tiles (instance, strip, row range) grouped by wave: wave
-// w of the persistent grid (its logical index after the XCD-aware remap of its workgroup)
-// marches tiles[first[w] .. first[w + 1]), a list the host plans so that every wave carries
-// about the same work and consecutive waves, which share an XCD, take neighbouring tiles. The
-// kernel's arguments are table pointers only: the host may add and remove instances between
-// launches (by copies on the same stream) and a captured graph stays valid.
+// The work list is a device table of tiles (instance, strip, row range), planned by the host.
+// Queued hand-out (the default): one queue of tiles per XCD, which the workgroups on that XCD
+// take four neighbouring tiles at a time (see the loop below). Static hand-out (the capacity
+// fill): wave w of the persistent grid (its logical index after the XCD-aware remap of its
+// workgroup) marches tiles[first[w] .. first[w + 1]). The kernel's arguments are table
+// pointers only: the host may add and remove instances between launches (by copies on the
+// same stream) and a captured graph stays valid.
 #pragma once
 
 #include "mhd_batch_plan.hpp"
