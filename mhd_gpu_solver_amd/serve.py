@@ -96,11 +96,12 @@ class ServeReport:
 
 
 class Server:
-    # cells stepping per replay under srpt. On MI355X (4096 fp32 wave slots, 60 columns each)
-    # 8M is about one round of the persistent grid at 32 rows per wave: the request stream of
-    # 128 solves at 200/s keeps its full throughput down to about 6M (4M: -5 %), while the
-    # median latency falls with the cap (16M: 1.25 s, 8M: 0.74 s, 6M: 0.60 s).
-    DEFAULT_WORK_CAP = 8 << 20
+    # Cells stepping per replay under srpt. With tiles handed out from per-XCD queues, the
+    # request stream of 128 solves at 200/s (FP32, MI355X) keeps its throughput down to about
+    # 5M cells and loses 5 % at 4M, while the median latency falls with the cap: 8M 0.67-0.72 s at 19.0-19.2 G cell-updates/s, p99
+    # 6.39-6.42 s; 6M 0.58 s, 19.4 G, p99 6.19 s; 5M 0.48 s, 19.4 G, p99 6.23 s; 4M 0.48 s,
+    # 18.3 G, p99 6.38 s.
+    DEFAULT_WORK_CAP = 5 << 20
 
     def __init__(self, precision="fp32", limiter=SlopeLimiter.MC, hbm_budget=32 << 30,
                  max_instances=1024, cfl=0.4, glm_alpha=0.1, policy="srpt",
