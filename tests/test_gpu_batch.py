@@ -196,10 +196,12 @@ def test_batch_half_wave_tiles_edges_match_oracle(hip, oracle, rows):
         assert np.array_equal(eng.download(i), Uo), (g.nx, g.ny, bc, rows)
 
 
-def test_batch_fp32_narrow_grids_tiling_independent_and_within_l1(hip, oracle):
-    """The fp32 engine on narrow and short grids (tails of 1 to 28 columns, 1 to 3 rows): the
-    same bits for every rows-per-wave choice, and within the fp32 bar (1e-4 relative L1 per
-    field) of the oracle's fp64 result."""
+@pytest.mark.parametrize("precision", ["fp32", "fp64_fast"])
+def test_batch_fp32_narrow_grids_tiling_independent_and_within_l1(hip, oracle, precision):
+    """The fp32 engine (and the contracted fp64 one, whose odd bands march the other way) on
+    narrow and short grids (tails of 1 to 28 columns, 1 to 3 rows): the same bits for every
+    rows-per-wave choice, and within the fp32 bar (1e-4 relative L1 per field) of the
+    oracle's fp64 result."""
     cases = [("orszag_tang", 61, 1, (PERIODIC, PERIODIC)),
              ("rotor", 88, 2, (OUTFLOW, OUTFLOW)),
              ("orszag_tang", 28, 3, (PERIODIC, OUTFLOW)),
@@ -209,7 +211,7 @@ def test_batch_fp32_narrow_grids_tiling_independent_and_within_l1(hip, oracle):
               for name, nx, ny, _ in cases]
     first = None
     for rows in (0, 1, 2, 200, -1):
-        eng = hip.BatchEngine("fp32", limiter=hip.SlopeLimiter.MC, hbm_budget=1 << 28)
+        eng = hip.BatchEngine(precision, limiter=hip.SlopeLimiter.MC, hbm_budget=1 << 28)
         if rows:
             eng.set_seg_rows(rows)
         ids = [eng.add_steps(U0, 7, bc=c[3]) for U0, c in zip(states, cases)]
